@@ -1,0 +1,88 @@
+// Device data structures and kernel launchers shared by the host runtime and kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace ctlk {
+
+enum class Storage : int { F32 = 0, F16 = 1, BF16 = 2 };
+
+inline size_t storageBytes(Storage s) { return s == Storage::F32 ? 4 : 2; }
+
+// A resident volume in HBM: (X+2) x (Y+2) x (Z+2) texels, x fastest, with a one-texel zero
+// border on every side, so that every tap of a sample inside the support is an in-bounds read.
+struct VolumeView {
+    const void* data;
+    uint32_t n[3];      // interior voxel counts
+    uint32_t pitch_y;   // X + 2        (texels per row)
+    uint32_t pitch_z;   // (X+2)*(Y+2)  (texels per slice)
+    uint32_t bytes;     // total bytes of the padded array (buffer range for the gather loads)
+    Storage storage;
+};
+
+// Volume geometry after upsampling, as the host passes it (FP32, raycasterprojector.cpp:169-179).
+struct VolumeGeom {
+    float voxel_mm[3];
+    float offset_mm[3];
+    uint32_t n[3];
+    float increment_mm; // min(voxel size) * raySampling
+};
+
+// Per (view, module) ray geometry written by the FP64 setup kernel. The ray direction of pixel
+// (x, y) is d = a*(x - xr) + b*(y - yr) + o (any positive scale); `c` is the source in the
+// padded texel frame: (source - corner)/voxel + 1/2.
+struct alignas(16) RayGeom {
+    float a[3];
+    float b[3];
+    float o[3];
+    float xr, yr;
+    float c[3];
+    uint32_t valid; // 0 if M is singular
+    float pad;
+};
+static_assert(sizeof(RayGeom) == 64, "RayGeom is one 64-byte record");
+
+struct LaunchShape {
+    uint32_t wave_w = 8, wave_h = 8; // pixel tile per wavefront (wave_w * wave_h = 64)
+    uint32_t block_waves_x = 2, block_waves_y = 2; // waves per workgroup
+    uint32_t unroll = 2;
+};
+
+struct RayParams {
+    uint32_t channels, rows, modules, views;
+    uint32_t rpp_x, rpp_y;
+    int interpolate;
+};
+
+// FP64 setup: one thread per (view, module).
+hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,
+                       uint32_t rows, const VolumeGeom& vg, int interpolate, RayGeom* d_geom,
+                       hipStream_t s);
+
+// Ray casting of all views in one launch; out is [views][modules][rows][channels].
+hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
+                         const RayParams& rp, const LaunchShape& shape, float* d_out,
+                         hipStream_t s);
+
+// Reference loop-count of all rays, added into *d_count.
+hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
+                       unsigned long long* d_count, hipStream_t s);
+
+// float (X,Y,Z) -> padded storage, replicating each voxel f^3 times (f >= 1).
+hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, void* d_dst,
+                      hipStream_t s);
+// padded storage -> padded storage upsampled by f (f >= 1).
+hipError_t launchUpsample(const VolumeView& src, uint32_t f, void* d_dst, hipStream_t s);
+// Mean of the interior of a padded volume, into *d_sum (double, accumulated).
+hipError_t launchSum(const VolumeView& src, double* d_sum, hipStream_t s);
+// Fill a one-texel padded volume (1x1x1 interior) with the value *d_sum / count.
+hipError_t launchFillMean(const double* d_sum, double count, Storage st, void* d_dst,
+                          hipStream_t s);
+// Zero-fill of a float array with a plain kernel (graph-capture safe).
+hipError_t launchZero(float* d, size_t n, hipStream_t s);
+
+bool validShape(const LaunchShape& s);
+
+} // namespace ctlk

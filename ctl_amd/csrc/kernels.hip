@@ -1,0 +1,625 @@
+// HIP kernels of the forward projector, written for gfx950 (CDNA4, wave64).
+//
+//   setup_kernel    FP64, one thread per (view, module): source position (Cramer's rule) and
+//                   sign-corrected M^-1, folded into an FP32 per-pixel direction basis.
+//   raycast_kernel  one thread per detector pixel, wavefront = 8x8 (tunable) pixel tile; FP32
+//                   slab clip, fixed-step trilinear samples gathered from the padded HBM volume
+//                   (FP32 / FP16 / BF16 storage), FP32 interpolation and accumulation.
+//   count_kernel    the reference's sampling-loop count of every ray (for Gsamples/s).
+//   pack / upsample / sum / fill kernels for resident volumes.
+//
+// Gathers use raw buffer loads: an offset past the end of the volume returns 0 instead of
+// touching memory, so no index can fault even if a position were computed wrongly.
+#include "kernels.h"
+
+#include <cfloat>
+#include <cstdint>
+
+namespace ctlk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNumXcd = 8; // MI355X: 8 XCDs, workgroups dealt round-robin
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// buffer resource word 3 for gfx9-family raw buffers
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t makeRsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, kRsrcWord3);
+}
+
+// Workgroup b of nb is dispatched to XCD b % 8; give each XCD a contiguous range of tiles so that
+// neighbouring tiles (which gather the same voxels) share that XCD's L2.
+__device__ __forceinline__ uint32_t xcdRemap(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    const uint32_t x = b % kNumXcd, k = b / kNumXcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 setup
+
+__device__ double det3(const double* m)
+{
+    return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
+           m[2] * (m[3] * m[7] - m[6] * m[4]);
+}
+
+__global__ void setup_kernel(const double* __restrict__ pm, uint32_t views, uint32_t modules,
+                             uint32_t channels, uint32_t rows, VolumeGeom vg, double shift,
+                             RayGeom* __restrict__ out)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= views * modules) return;
+    const uint32_t v = t / modules;
+    const double* P = pm + size_t(t) * 12;
+    const double* P0 = pm + size_t(v) * modules * 12; // source from module 0 (raycasterprojector.cpp:323-325)
+
+    // source = null space of P0 by Cramer's rule (projectionmatrix.cpp:264-274)
+    double m[9];
+    const int cols[4][3] = { { 0, 1, 2 }, { 1, 2, 3 }, { 0, 2, 3 }, { 0, 1, 3 } };
+    double dets[4];
+    for (int k = 0; k < 4; ++k) {
+        for (int r = 0; r < 3; ++r)
+            for (int j = 0; j < 3; ++j) m[3 * r + j] = P0[4 * r + cols[k][j]];
+        dets[k] = det3(m);
+    }
+    const double h = -dets[0];
+    const double src[3] = { dets[1] / h, -dets[2] / h, dets[3] / h };
+
+    // M^-1 of this module by the adjugate, times sign(det M): the direction sign rule of
+    // decomposeM (raycasterprojector.cpp:527-541, R negated when det R < 0, det Q = +1).
+    for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 3; ++j) m[3 * r + j] = P[4 * r + j];
+    const double det = det3(m);
+    double adj[9];
+    adj[0] = m[4] * m[8] - m[5] * m[7];
+    adj[1] = m[2] * m[7] - m[1] * m[8];
+    adj[2] = m[1] * m[5] - m[2] * m[4];
+    adj[3] = m[5] * m[6] - m[3] * m[8];
+    adj[4] = m[0] * m[8] - m[2] * m[6];
+    adj[5] = m[2] * m[3] - m[0] * m[5];
+    adj[6] = m[3] * m[7] - m[4] * m[6];
+    adj[7] = m[1] * m[6] - m[0] * m[7];
+    adj[8] = m[0] * m[4] - m[1] * m[3];
+    const double inv_abs_det = 1.0 / fabs(det);
+
+    const double xr = 0.5 * double(channels - 1), yr = 0.5 * double(rows - 1);
+    double a[3], b[3], o[3];
+    for (int r = 0; r < 3; ++r) {
+        a[r] = adj[3 * r + 0] * inv_abs_det;
+        b[r] = adj[3 * r + 1] * inv_abs_det;
+        o[r] = a[r] * xr + b[r] * yr + adj[3 * r + 2] * inv_abs_det;
+    }
+    const double on = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+    const bool ok = det != 0.0 && isfinite(det) && on > 0.0 && isfinite(h) && h != 0.0;
+
+    RayGeom g;
+    for (int r = 0; r < 3; ++r) {
+        g.a[r] = float(a[r] / on);
+        g.b[r] = float(b[r] / on);
+        g.o[r] = float(o[r] / on);
+        // volumeCorner (raycasterprojector.cpp:552-558), then the padded texel frame
+        const double corner = double(vg.offset_mm[r]) - 0.5 * double(vg.n[r]) * double(vg.voxel_mm[r]);
+        g.c[r] = float((src[r] - corner) / double(vg.voxel_mm[r]) + shift);
+    }
+    g.xr = float(xr);
+    g.yr = float(yr);
+    g.valid = ok ? 1u : 0u;
+    g.pad = 0.0f;
+    out[t] = g;
+}
+
+// ---------------------------------------------------------------------------------------------
+// per-ray setup (FP32)
+
+struct Ray {
+    float s[3]; // step per sample, texel units
+    int i0, i1; // inclusive sample index range (reference loop bounds)
+    bool hit;
+};
+
+// checkFace (raycasterprojector_interp.cl:136-148)
+__device__ __forceinline__ void checkFace(float h0, float h1, float c10, float c11, float c20,
+                                          float c21, float lambda, float& lo, float& hi)
+{
+    const bool in = (h0 >= c10) & (h1 >= c11) & (h0 <= c20) & (h1 <= c21);
+    lo = (in & (lambda < lo)) ? lambda : lo;
+    hi = (in & (lambda > hi)) ? lambda : hi;
+}
+
+// Direction, step and the reference's sample range for the sub-ray through detector
+// coordinate (px, py). The slab test restates calculateIntersections
+// (raycasterprojector_interp.cl:88-134) in the texel frame whose box is [blo, n + 1].
+__device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, float inc,
+                                        const float vs[3], const float nhi[3], float blo)
+{
+    Ray r;
+    const float ex = px - g.xr, ey = py - g.yr;
+    float d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = fmaf(g.a[k], ex, fmaf(g.b[k], ey, g.o[k]));
+    const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r.s[k] = (inc * (d[k] / len)) / vs[k];
+
+    float c1[3], c2[3], l1[3], l2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c1[k] = blo;
+        c2[k] = nhi[k];
+        l1[k] = (c1[k] - g.c[k]) / r.s[k];
+        l2[k] = (c2[k] - g.c[k]) / r.s[k];
+        c1[k] -= 0.01f;
+        c2[k] += 0.01f;
+    }
+    float lo = FLT_MAX, hi = 0.0f;
+    const float* c = g.c;
+    const float* s = r.s;
+    checkFace(c[1] + l1[0] * s[1], c[2] + l1[0] * s[2], c1[1], c1[2], c2[1], c2[2], l1[0], lo, hi);
+    checkFace(c[0] + l1[1] * s[0], c[2] + l1[1] * s[2], c1[0], c1[2], c2[0], c2[2], l1[1], lo, hi);
+    checkFace(c[0] + l1[2] * s[0], c[1] + l1[2] * s[1], c1[0], c1[1], c2[0], c2[1], l1[2], lo, hi);
+    checkFace(c[1] + l2[0] * s[1], c[2] + l2[0] * s[2], c1[1], c1[2], c2[1], c2[2], l2[0], lo, hi);
+    checkFace(c[0] + l2[1] * s[0], c[2] + l2[1] * s[2], c1[0], c1[2], c2[0], c2[2], l2[1], lo, hi);
+    checkFace(c[0] + l2[2] * s[0], c[1] + l2[2] * s[1], c1[0], c1[1], c2[0], c2[1], l2[2], lo, hi);
+    r.hit = lo != FLT_MAX;
+    lo = fmaxf(lo, 0.0f);
+    hi = fmaxf(hi, 0.0f);
+    // (uint) conversions of the loop bounds; lambda stays far below 2^30 for any real geometry,
+    // the clamp only keeps the integer arithmetic defined.
+    r.i0 = int(fminf(lo, 1073741824.0f));
+    r.i1 = int(fminf(hi, 1073741824.0f)) + 1;
+    return r;
+}
+
+// Samples whose position lies outside [blo, n+1) on any axis read only border texels and add
+// exactly 0; drop them from both ends so the hot loop needs no bounds test. Positions are
+// monotone in i on every axis, so the remaining range is contiguous.
+__device__ __forceinline__ bool insideQ(int i, const float s[3], const float c[3], float blo,
+                                        const float nhi[3])
+{
+    const float fi = float(i);
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float q = fmaf(fi, s[k], c[k]);
+        in = in & (q >= blo) & (q < nhi[k]);
+    }
+    return in;
+}
+
+__device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, const float nhi[3])
+{
+    if (!r.hit) { r.i0 = 1; r.i1 = 0; return; }
+    while (r.i0 <= r.i1 && !insideQ(r.i0, r.s, c, blo, nhi)) ++r.i0;
+    while (r.i1 >= r.i0 && !insideQ(r.i1, r.s, c, blo, nhi)) --r.i1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// trilinear sample at texel-frame position q (padded frame, 0 <= q < n+1): x-adjacent taps are
+// one 8-byte (FP32) or 4-byte (FP16/BF16) load; four loads per sample.
+
+struct Pitch {
+    uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) rows
+    uint32_t ty, tz;   // texels per row / slice
+};
+
+template <Storage ST> struct Taps;
+
+template <> struct Taps<Storage::F32> {
+    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    {
+        const auto raw = __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0);
+        const float2 t = __builtin_bit_cast(float2, raw);
+        return fmaf(w, t.y - t.x, t.x);
+    }
+    static constexpr uint32_t kShift = 2;
+};
+
+template <> struct Taps<Storage::F16> {
+    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    {
+        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, so, 0);
+        const half2_t h = __builtin_bit_cast(half2_t, raw);
+        const float a = float(h.x), b = float(h.y);
+        return fmaf(w, b, fmaf(-w, a, a));
+    }
+    static constexpr uint32_t kShift = 1;
+};
+
+template <> struct Taps<Storage::BF16> {
+    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    {
+        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, so, 0);
+        const float a = __uint_as_float(raw << 16), b = __uint_as_float(raw & 0xffff0000u);
+        return fmaf(w, b - a, a);
+    }
+    static constexpr uint32_t kShift = 1;
+};
+
+template <Storage ST>
+__device__ __forceinline__ float trilinear(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
+                                           float qy, float qz)
+{
+    const int ix = int(qx), iy = int(qy), iz = int(qz); // q >= 0: truncation is floor
+    const float wx = __builtin_amdgcn_fractf(qx);
+    const float wy = __builtin_amdgcn_fractf(qy);
+    const float wz = __builtin_amdgcn_fractf(qz);
+    const uint32_t idx = uint32_t(ix) + __umul24(uint32_t(iy), p.ty) + __umul24(uint32_t(iz), p.tz);
+    const uint32_t ofs = idx << Taps<ST>::kShift;
+    const float c00 = Taps<ST>::lerpx(r, ofs, 0, wx);
+    const float c10 = Taps<ST>::lerpx(r, ofs, p.y, wx);
+    const float c01 = Taps<ST>::lerpx(r, ofs, p.z, wx);
+    const float c11 = Taps<ST>::lerpx(r, ofs, p.yz, wx);
+    const float c0 = fmaf(wy, c10 - c00, c00);
+    const float c1 = fmaf(wy, c11 - c01, c01);
+    return fmaf(wz, c1 - c0, c0);
+}
+
+template <Storage ST>
+__device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
+                                         float qy, float qz)
+{
+    const uint32_t idx = uint32_t(int(qx)) + __umul24(uint32_t(int(qy)), p.ty) +
+                         __umul24(uint32_t(int(qz)), p.tz);
+    if constexpr (ST == Storage::F32) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx << 2, 0, 0));
+    } else {
+        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b16(r, idx << 1, 0, 0);
+        if constexpr (ST == Storage::F16) {
+            const _Float16 h = __builtin_bit_cast(_Float16, uint16_t(raw));
+            return float(h);
+        } else {
+            return __uint_as_float(raw << 16);
+        }
+    }
+}
+
+struct CastConst {
+    uint32_t tiles_x, tiles_y, tiles_per_vm;
+    uint32_t wave_w, wave_h, bwx;
+    float inc;
+    float vs[3];
+    float nhi[3]; // n + 1 per axis
+    float blo;    // 0 (trilinear) or 1 (nearest)
+    Pitch pitch;
+};
+
+template <Storage ST, bool INTERP, int UNROLL>
+__global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
+                                                         RayParams rp, CastConst kc,
+                                                         float* __restrict__ out)
+{
+    const uint32_t b = xcdRemap(blockIdx.x, gridDim.x);
+    const uint32_t vm = b / kc.tiles_per_vm;
+    const uint32_t t = b - vm * kc.tiles_per_vm;
+    const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
+    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
+    const uint32_t y = (ty * (kBlock / 64 / kc.bwx) + wy) * kc.wave_h + ly;
+    if (x >= rp.channels || y >= rp.rows) return;
+
+    const RayGeom g = geom[vm];
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+
+    // kernel prologue of raycasterprojector_interp.cl:39-40
+    const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
+    const float pcx = float(x) - 0.5f + 0.5f * ipx;
+    const float pcy = float(y) - 0.5f + 0.5f * ipy;
+
+    float acc = 0.0f;
+    if (g.valid) {
+        for (uint32_t ra = 0; ra < rp.rpp_x; ++ra)
+            for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
+                Ray ray = setupRay(g, pcx + float(ra) * ipx, pcy + float(rb) * ipy, kc.inc, kc.vs,
+                                   kc.nhi, kc.blo);
+                tighten(ray, g.c, kc.blo, kc.nhi);
+                int n = ray.i1 - ray.i0 + 1;
+                float fi = float(ray.i0);
+                float part[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
+                for (; n >= UNROLL; n -= UNROLL, fi += float(UNROLL)) {
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) {
+                        const float fu = fi + float(u);
+                        const float qx = fmaf(fu, ray.s[0], g.c[0]);
+                        const float qy = fmaf(fu, ray.s[1], g.c[1]);
+                        const float qz = fmaf(fu, ray.s[2], g.c[2]);
+                        part[u] += INTERP ? trilinear<ST>(rs, kc.pitch, qx, qy, qz)
+                                          : nearest<ST>(rs, kc.pitch, qx, qy, qz);
+                    }
+                }
+                for (; n > 0; --n, fi += 1.0f) {
+                    const float qx = fmaf(fi, ray.s[0], g.c[0]);
+                    const float qy = fmaf(fi, ray.s[1], g.c[1]);
+                    const float qz = fmaf(fi, ray.s[2], g.c[2]);
+                    part[0] += INTERP ? trilinear<ST>(rs, kc.pitch, qx, qy, qz)
+                                      : nearest<ST>(rs, kc.pitch, qx, qy, qz);
+                }
+                float sum = 0.0f;
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) sum += part[u];
+                acc += sum;
+            }
+    }
+    // raycasterprojector_interp.cl:70-71
+    out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+}
+
+__global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict__ geom, RayParams rp,
+                                                       CastConst kc, unsigned long long* count)
+{
+    __shared__ unsigned long long block_sum;
+    if (threadIdx.x == 0) block_sum = 0;
+    __syncthreads();
+    const size_t pix = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t per_vm = size_t(rp.channels) * rp.rows;
+    unsigned long long mine = 0;
+    if (pix < per_vm * rp.views * rp.modules) {
+        const uint32_t vm = uint32_t(pix / per_vm);
+        const uint32_t rem = uint32_t(pix - size_t(vm) * per_vm);
+        const uint32_t y = rem / rp.channels, x = rem - y * rp.channels;
+        const RayGeom g = geom[vm];
+        const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
+        const float pcx = float(x) - 0.5f + 0.5f * ipx;
+        const float pcy = float(y) - 0.5f + 0.5f * ipy;
+        if (g.valid)
+            for (uint32_t ra = 0; ra < rp.rpp_x; ++ra)
+                for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
+                    const Ray ray = setupRay(g, pcx + float(ra) * ipx, pcy + float(rb) * ipy, kc.inc,
+                                             kc.vs, kc.nhi, kc.blo);
+                    if (ray.hit && ray.i1 >= ray.i0) mine += (unsigned long long)(ray.i1 - ray.i0 + 1);
+                }
+    }
+    if (mine) atomicAdd(&block_sum, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_sum) atomicAdd(count, block_sum);
+}
+
+// ---------------------------------------------------------------------------------------------
+// volume conversion
+
+__device__ __forceinline__ uint16_t toHalfBits(float f)
+{
+    return __builtin_bit_cast(uint16_t, _Float16(f)); // round to nearest even
+}
+
+__device__ __forceinline__ uint16_t toBf16Bits(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+    return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ void storeTexel(void* dst, size_t i, float v, Storage st)
+{
+    if (st == Storage::F32) static_cast<float*>(dst)[i] = v;
+    else if (st == Storage::F16) static_cast<uint16_t*>(dst)[i] = toHalfBits(v);
+    else static_cast<uint16_t*>(dst)[i] = toBf16Bits(v);
+}
+
+__device__ __forceinline__ float loadTexel(const void* src, size_t i, Storage st)
+{
+    if (st == Storage::F32) return static_cast<const float*>(src)[i];
+    const uint16_t h = static_cast<const uint16_t*>(src)[i];
+    if (st == Storage::F16) return float(__builtin_bit_cast(_Float16, h));
+    return __uint_as_float(uint32_t(h) << 16);
+}
+
+__global__ void pack_kernel(const float* __restrict__ src, uint32_t nx, uint32_t ny, uint32_t nz,
+                            uint32_t f, Storage st, void* __restrict__ dst)
+{
+    const size_t px = size_t(nx) * f + 2, py = size_t(ny) * f + 2, pz = size_t(nz) * f + 2;
+    const size_t total = px * py * pz;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const size_t x = i % px, yz = i / px, y = yz % py, z = yz / py;
+        float v = 0.0f;
+        if (x >= 1 && y >= 1 && z >= 1 && x <= px - 2 && y <= py - 2 && z <= pz - 2)
+            v = src[((z - 1) / f * ny + (y - 1) / f) * nx + (x - 1) / f];
+        storeTexel(dst, i, v, st);
+    }
+}
+
+__global__ void upsample_kernel(VolumeView s, uint32_t f, void* __restrict__ dst)
+{
+    const size_t px = size_t(s.n[0]) * f + 2, py = size_t(s.n[1]) * f + 2, pz = size_t(s.n[2]) * f + 2;
+    const size_t total = px * py * pz;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const size_t x = i % px, yz = i / px, y = yz % py, z = yz / py;
+        float v = 0.0f;
+        if (x >= 1 && y >= 1 && z >= 1 && x <= px - 2 && y <= py - 2 && z <= pz - 2) {
+            const size_t sx = (x - 1) / f + 1, sy = (y - 1) / f + 1, sz = (z - 1) / f + 1;
+            v = loadTexel(s.data, (sz * s.pitch_z) + sy * s.pitch_y + sx, s.storage);
+        }
+        storeTexel(dst, i, v, s.storage);
+    }
+}
+
+__global__ void sum_kernel(VolumeView s, double* sum)
+{
+    __shared__ double part[kBlock];
+    double acc = 0.0;
+    const size_t nx = s.n[0], ny = s.n[1], total = size_t(s.n[0]) * s.n[1] * s.n[2];
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const size_t x = i % nx, yz = i / nx, y = yz % ny, z = yz / ny;
+        acc += loadTexel(s.data, (z + 1) * s.pitch_z + (y + 1) * s.pitch_y + x + 1, s.storage);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(sum, part[0]);
+}
+
+__global__ void fill_mean_kernel(const double* sum, double count, Storage st, void* dst)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < 27) {
+        // the reference sums in float (raycasterprojector.cpp:603-606); the mean is rounded once
+        const float mean = float(*sum / count);
+        storeTexel(dst, i, i == 13 ? mean : 0.0f, st);
+    }
+}
+
+__global__ void zero_kernel(float* d, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x)
+        d[i] = 0.0f;
+}
+
+uint32_t gridFor(size_t n)
+{
+    const size_t blocks = (n + kBlock - 1) / kBlock;
+    return uint32_t(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
+}
+
+CastConst makeConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
+                    const LaunchShape& sh)
+{
+    CastConst kc{};
+    kc.wave_w = sh.wave_w;
+    kc.wave_h = sh.wave_h;
+    kc.bwx = sh.block_waves_x;
+    const uint32_t btw = sh.wave_w * sh.block_waves_x, bth = sh.wave_h * sh.block_waves_y;
+    kc.tiles_x = (rp.channels + btw - 1) / btw;
+    kc.tiles_y = (rp.rows + bth - 1) / bth;
+    kc.tiles_per_vm = kc.tiles_x * kc.tiles_y;
+    kc.inc = vg.increment_mm;
+    for (int k = 0; k < 3; ++k) {
+        kc.vs[k] = vg.voxel_mm[k];
+        kc.nhi[k] = float(vg.n[k]) + 1.0f;
+    }
+    kc.blo = rp.interpolate ? 0.0f : 1.0f;
+    if (vol) {
+        const uint32_t eb = uint32_t(storageBytes(vol->storage));
+        kc.pitch.ty = vol->pitch_y;
+        kc.pitch.tz = vol->pitch_z;
+        kc.pitch.y = vol->pitch_y * eb;
+        kc.pitch.z = vol->pitch_z * eb;
+        kc.pitch.yz = (vol->pitch_y + vol->pitch_z) * eb;
+    }
+    return kc;
+}
+
+template <Storage ST, bool INTERP>
+hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams& rp,
+                       const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
+                       hipStream_t s)
+{
+    switch (unroll) {
+    case 1: raycast_kernel<ST, INTERP, 1><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    case 4: raycast_kernel<ST, INTERP, 4><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    default: raycast_kernel<ST, INTERP, 2><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    }
+    return hipGetLastError();
+}
+
+template <Storage ST>
+hipError_t launchCastI(const VolumeView& vol, const RayGeom* g, const RayParams& rp,
+                       const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
+                       hipStream_t s)
+{
+    return rp.interpolate ? launchCastU<ST, true>(vol, g, rp, kc, unroll, blocks, out, s)
+                          : launchCastU<ST, false>(vol, g, rp, kc, unroll, blocks, out, s);
+}
+
+} // namespace
+
+bool validShape(const LaunchShape& s)
+{
+    return s.wave_w * s.wave_h == 64 && s.block_waves_x * s.block_waves_y == kBlock / 64 &&
+           (s.unroll == 1 || s.unroll == 2 || s.unroll == 4);
+}
+
+hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,
+                       uint32_t rows, const VolumeGeom& vg, int interpolate, RayGeom* d_geom,
+                       hipStream_t s)
+{
+    // texel frame of the gathers: trilinear q = p + 1/2 (centre shift -1/2, padding +1),
+    // nearest q = p + 1 (padding +1), p = position relative to the volume corner in voxels
+    const double shift = interpolate ? 0.5 : 1.0;
+    const uint32_t n = views * modules;
+    setup_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_pmats, views, modules, channels, rows, vg, shift,
+                                              d_geom);
+    return hipGetLastError();
+}
+
+hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
+                         const RayParams& rp, const LaunchShape& shape, float* d_out,
+                         hipStream_t s)
+{
+    if (!validShape(shape)) return hipErrorInvalidValue;
+    const CastConst kc = makeConst(&vol, vg, rp, shape);
+    const uint64_t blocks = uint64_t(kc.tiles_per_vm) * rp.views * rp.modules;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
+    switch (vol.storage) {
+    case Storage::F16: return launchCastI<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
+    case Storage::BF16: return launchCastI<Storage::BF16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
+    default: return launchCastI<Storage::F32>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
+    }
+}
+
+hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
+                       unsigned long long* d_count, hipStream_t s)
+{
+    const CastConst kc = makeConst(nullptr, vg, rp, LaunchShape{});
+    const uint64_t rays = uint64_t(rp.channels) * rp.rows * rp.modules * rp.views;
+    const uint64_t blocks = (rays + kBlock - 1) / kBlock;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
+    count_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_geom, rp, kc, d_count);
+    return hipGetLastError();
+}
+
+hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, void* d_dst,
+                      hipStream_t s)
+{
+    const size_t total = (size_t(n[0]) * f + 2) * (size_t(n[1]) * f + 2) * (size_t(n[2]) * f + 2);
+    pack_kernel<<<gridFor(total), kBlock, 0, s>>>(d_src, n[0], n[1], n[2], f, st, d_dst);
+    return hipGetLastError();
+}
+
+hipError_t launchUpsample(const VolumeView& src, uint32_t f, void* d_dst, hipStream_t s)
+{
+    const size_t total =
+        (size_t(src.n[0]) * f + 2) * (size_t(src.n[1]) * f + 2) * (size_t(src.n[2]) * f + 2);
+    upsample_kernel<<<gridFor(total), kBlock, 0, s>>>(src, f, d_dst);
+    return hipGetLastError();
+}
+
+hipError_t launchSum(const VolumeView& src, double* d_sum, hipStream_t s)
+{
+    const size_t total = size_t(src.n[0]) * src.n[1] * src.n[2];
+    sum_kernel<<<gridFor(total) < 4096 ? gridFor(total) : 4096, kBlock, 0, s>>>(src, d_sum);
+    return hipGetLastError();
+}
+
+hipError_t launchFillMean(const double* d_sum, double count, Storage st, void* d_dst, hipStream_t s)
+{
+    fill_mean_kernel<<<1, 64, 0, s>>>(d_sum, count, st, d_dst);
+    return hipGetLastError();
+}
+
+hipError_t launchZero(float* d, size_t n, hipStream_t s)
+{
+    zero_kernel<<<gridFor(n), kBlock, 0, s>>>(d, n);
+    return hipGetLastError();
+}
+
+} // namespace ctlk

@@ -1,0 +1,196 @@
+/*
+ * ctlproj.h -- C ABI of the MI355X forward projector (drop-in for CTL::OCL::RayCasterProjector).
+ *
+ * Every entry point returns a ctlproj_status; no exception crosses the ABI. The message of the
+ * last failure on the calling thread is available from ctlproj_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the CTL source tree):
+ *   OpenCLConfig / ClFileLoader / PinnedBuf*      modules/src/ocl/openclconfig.h:38-131,
+ *                                                 clfileloader.h:47-77, pinnedmem.h:143-173
+ *   RayCasterProjector::Settings                  modules/src/projectors/raycasterprojector.h:27-37
+ *   RayCasterProjector::configure                 modules/src/projectors/raycasterprojector.cpp:97-107
+ *   RayCasterProjector::project                   modules/src/projectors/raycasterprojector.cpp:139-375
+ *   GeometryEncoder::encodeFullGeometry           modules/src/acquisition/geometryencoder.cpp:83-135
+ *
+ * Data layouts:
+ *   volume      float, x fastest: index = (z*Y + y)*X + x        (voxelvolume.tpp:310-332)
+ *   P-matrices  double [views][modules][3][4], row-major          (viewgeometry.h:21-102)
+ *   projections float [views][modules][rows][channels]           (projectiondata.cpp:134-228)
+ */
+#ifndef CTLPROJ_H
+#define CTLPROJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ctlproj_status {
+    CTLPROJ_OK = 0,
+    CTLPROJ_ERR_INVALID_ARGUMENT = 1, /* bad pointer, size or setting */
+    CTLPROJ_ERR_NO_DEVICE = 2,        /* no HIP device, or an invalid device ordinal */
+    CTLPROJ_ERR_DEVICE = 3,           /* a HIP runtime call failed */
+    CTLPROJ_ERR_OUT_OF_MEMORY = 4,    /* device or host allocation failed */
+    CTLPROJ_ERR_NOT_CONFIGURED = 5,   /* project() before configure() */
+    CTLPROJ_ERR_NO_DATA = 6,          /* volume without data (reference returns 0 views) */
+    CTLPROJ_ERR_OVER_BUDGET = 7,      /* a request larger than the server's HBM budget */
+    CTLPROJ_ERR_INJECTED = 8          /* failure forced by ctlproj_debug_inject_failure */
+} ctlproj_status;
+
+typedef enum ctlproj_dtype {
+    CTLPROJ_F32 = 0,
+    CTLPROJ_F16 = 1,
+    CTLPROJ_BF16 = 2
+} ctlproj_dtype;
+
+/* RayCasterProjector::Settings (raycasterprojector.h:27-37) plus tuning fields that the reference
+ * does not have. ctlproj_settings_default() fills the reference defaults: rays per pixel {1,1},
+ * ray sampling 0.3, upsampling 1, interpolate on; tuning fields 0 = automatic. */
+typedef struct ctlproj_settings {
+    uint32_t rays_per_pixel[2];
+    float ray_sampling;
+    uint32_t volume_upsampling;
+    int32_t interpolate;
+    uint32_t tile_w;   /* pixels per wavefront tile along channels (0 = auto) */
+    uint32_t tile_h;   /* pixels per wavefront tile along rows (0 = auto) */
+    uint32_t unroll;   /* ray-step unroll of the sampling loop (0 = auto) */
+} ctlproj_settings;
+
+typedef struct ctlproj_context ctlproj_context;
+typedef struct ctlproj_volume ctlproj_volume;
+typedef struct ctlproj_projector ctlproj_projector;
+
+/* Per-view progress, the analogue of ProjectorNotifier::projectionFinished(int)
+ * (abstractprojector.h:57-63). Called on the calling thread once a view's data is in `out`. */
+typedef void (*ctlproj_progress_fn)(int view, void* user);
+
+/* ---- environment (replaces OpenCLConfig, openclconfig.cpp:15-171) ----------------------- */
+
+void ctlproj_settings_default(ctlproj_settings* s);
+const char* ctlproj_last_error(void);
+ctlproj_status ctlproj_device_count(int* count);
+ctlproj_status ctlproj_context_create(int device, ctlproj_context** out);
+ctlproj_status ctlproj_context_destroy(ctlproj_context* ctx);
+
+/* ---- resident volumes (replaces the per-call Image3D upload, raycasterprojector.cpp:236-251) */
+
+/* Uploads `host` (float, X*Y*Z) and keeps it resident in HBM in `storage` precision.
+ * voxel_mm: voxel size, offset_mm: volume offset (VoxelVolume::offset). */
+ctlproj_status ctlproj_volume_create(ctlproj_context* ctx, const float* host, const uint32_t dims[3],
+                                     const float voxel_mm[3], const float offset_mm[3],
+                                     ctlproj_dtype storage, ctlproj_volume** out);
+/* Same, from a float array already in device memory on the context's device. */
+ctlproj_status ctlproj_volume_create_from_device(ctlproj_context* ctx, const float* device_ptr,
+                                                 const uint32_t dims[3], const float voxel_mm[3],
+                                                 const float offset_mm[3], ctlproj_dtype storage,
+                                                 ctlproj_volume** out);
+ctlproj_status ctlproj_volume_destroy(ctlproj_volume* vol);
+/* Device bytes held by the volume. */
+ctlproj_status ctlproj_volume_bytes(const ctlproj_volume* vol, size_t* bytes);
+
+/* ---- projector (RayCasterProjector, raycasterprojector.h:20-58) -------------------------- */
+
+ctlproj_status ctlproj_projector_create(ctlproj_context* ctx, const ctlproj_settings* settings,
+                                        ctlproj_projector** out);
+ctlproj_status ctlproj_projector_destroy(ctlproj_projector* p);
+ctlproj_status ctlproj_projector_set_settings(ctlproj_projector* p, const ctlproj_settings* s);
+ctlproj_status ctlproj_projector_get_settings(const ctlproj_projector* p, ctlproj_settings* s);
+
+/* configure (raycasterprojector.cpp:97-107): `pmats` holds views*modules FP64 3x4 matrices;
+ * channels and rows are the pixels per module. The matrices are copied. */
+ctlproj_status ctlproj_projector_configure(ctlproj_projector* p, const double* pmats, uint32_t views,
+                                           uint32_t modules, uint32_t channels, uint32_t rows);
+
+/* project (raycasterprojector.cpp:139-375) into host memory: out holds
+ * views*modules*rows*channels floats. Blocks until done. `progress` may be NULL. */
+ctlproj_status ctlproj_project(ctlproj_projector* p, const ctlproj_volume* vol, float* out,
+                               size_t out_count, ctlproj_progress_fn progress, void* user);
+
+/* Reference-style one-shot project(VolumeData): uploads the host volume (FP32 storage),
+ * projects into host memory and frees the device copy. */
+ctlproj_status ctlproj_project_host_volume(ctlproj_projector* p, const float* host_volume,
+                                           const uint32_t dims[3], const float voxel_mm[3],
+                                           const float offset_mm[3], float* out, size_t out_count);
+
+/* Asynchronous projection into device memory on the projector's stream (no host copy).
+ * Call ctlproj_projector_synchronize before reading `device_out`. */
+ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume* vol,
+                                      float* device_out, size_t out_count);
+ctlproj_status ctlproj_projector_synchronize(ctlproj_projector* p);
+
+/* Duration of the last ray-casting kernel on the projector's stream (HIP events), in ms. */
+ctlproj_status ctlproj_projector_last_kernel_ms(ctlproj_projector* p, float* ms);
+
+/* Number of sampling-loop iterations the reference performs for this volume and geometry
+ * (sum over rays of floor(lambda_max) + 2 - floor(lambda_min)), counted on the device. */
+ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlproj_volume* vol,
+                                               uint64_t* n_samples);
+
+/* ---- geometry front end (GeometryEncoder, geometryencoder.cpp:83-135) -------------------- */
+
+typedef struct ctlproj_detector {
+    uint32_t pixels[2];       /* channels, rows per module */
+    double pixel_mm[2];       /* pixel width, height */
+    uint32_t modules;         /* 1 and cylindrical = 0: FlatPanelDetector */
+    int32_t cylindrical;      /* CylindricalDetector (cylindricaldetector.cpp:29-41) */
+    double angulation;        /* rad between adjacent modules */
+    double module_spacing;    /* mm */
+    double skew;
+} ctlproj_detector;
+
+typedef enum ctlproj_gantry_type { CTLPROJ_CARM = 0, CTLPROJ_TUBULAR = 1 } ctlproj_gantry_type;
+
+typedef struct ctlproj_gantry {
+    int32_t type;             /* ctlproj_gantry_type */
+    double carm_span;         /* CarmGantry(cArmSpan), carmgantry.cpp:12-16 */
+    double sdd, sid;          /* TubularGantry(SDD, SID, rotation, pitch, tilt), */
+    double rotation, pitch, tilt; /*   tubulargantry.cpp:20-33 */
+} ctlproj_gantry;
+
+typedef enum ctlproj_trajectory_type {
+    CTLPROJ_SHORT_SCAN = 0,       /* p = {source_to_iso, start_angle, angle_span (<0: 180deg+fan)} */
+    CTLPROJ_WOBBLE = 1,           /* p = {angle_span, source_to_iso, start_angle, wobble_angle, freq} */
+    CTLPROJ_CIRCLE_PLUS_LINE = 2, /* p = {angle_span, source_to_iso, line_length, fraction, start} */
+    CTLPROJ_HELICAL = 3,          /* p = {angle_increment, pitch_increment, start_pitch, start_angle} */
+    CTLPROJ_AXIAL = 4             /* p = {start_angle} */
+} ctlproj_trajectory_type;
+
+typedef struct ctlproj_trajectory {
+    int32_t type;
+    double p[6];
+} ctlproj_trajectory;
+
+/* encodeFullGeometry for a system and a trajectory protocol (trajectories.cpp). out holds
+ * views*modules*12 doubles. */
+ctlproj_status ctlproj_encode_trajectory(const ctlproj_gantry* gantry, const ctlproj_detector* det,
+                                         const ctlproj_trajectory* traj, uint32_t views,
+                                         double* out, size_t out_count);
+
+/* C-arm views at explicit source locations (prepare::CarmGantryParam::setLocation,
+ * preparesteps.cpp:71-84): positions [views][3] mm, rotations [views][3][3] row-major. */
+ctlproj_status ctlproj_encode_carm_poses(double carm_span, const ctlproj_detector* det,
+                                         const double* positions, const double* rotations,
+                                         uint32_t views, double* out, size_t out_count);
+
+/* ---- synthetic phantoms ------------------------------------------------------------------ */
+
+/* Seeded random-ellipsoid phantom (host memory, x fastest), see the generator description in
+ * ctl_amd/csrc/phantom.cpp. */
+ctlproj_status ctlproj_phantom_ellipsoids(const uint32_t dims[3], const float voxel_mm[3],
+                                          uint32_t n_ellipsoids, uint64_t seed,
+                                          float semi_axis_min_mm, float semi_axis_max_mm,
+                                          float mu_min, float mu_max, float background_mu,
+                                          float* out);
+
+/* ---- failure injection (tests of the error paths) --------------------------------------- */
+
+/* The next `count` device operations of the given context fail with CTLPROJ_ERR_INJECTED. */
+ctlproj_status ctlproj_debug_inject_failure(ctlproj_context* ctx, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CTLPROJ_H */

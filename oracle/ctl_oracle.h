@@ -1,0 +1,78 @@
+/*
+ * ctl_oracle.h -- CPU oracle for CTL's ray-caster forward projector.
+ *
+ * TEST INFRASTRUCTURE, NOT PRODUCT CODE. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker (or as the timed
+ * CPU baseline). The HIP product path never calls it.
+ *
+ * Two restatements of the reference algorithm live here:
+ *   ref32 -- the OpenCL kernel's numerics as a CPU OpenCL runtime executes them
+ *            (modules/src/ocl/cl_src/projectors/raycasterprojector_interp.cl:18-148 and
+ *             raycasterprojector_no_interp.cl:16-154, host side
+ *             modules/src/projectors/raycasterprojector.cpp:139-375, 527-672);
+ *   ref64 -- the same sample positions and loop bounds with every quantity in FP64
+ *            (the "FP64 CPU ray-caster" that error statements are made against).
+ * Plus the FP64 geometry helpers the host side uses (QR of M, source position).
+ */
+#ifndef CTL_ORACLE_H
+#define CTL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_settings {
+    uint32_t rays_per_pixel[2]; /* raycasterprojector.h:31 */
+    float ray_sampling;         /* raycasterprojector.h:32 */
+    uint32_t volume_upsampling; /* raycasterprojector.h:33 */
+    int32_t interpolate;        /* raycasterprojector.h:34 */
+} oracle_settings;
+
+/* Householder QR of a 3x3 matrix (row-major), restating mat::QRdecomposition
+ * (modules/src/mat/matrix_algorithm.cpp:14-37) and qrdcmp (:125-171). Returns 1 if singular. */
+int oracle_qr(const double A[9], double Q[9], double R[9]);
+
+/* decomposeM (raycasterprojector.cpp:527-541): QR of M = P[:, 0:3] with R negated when
+ * R00*R11*R22 < 0, packed as [Q row-major (9), R00, R01, R02, R11, R12, R22, 0]. */
+void oracle_decompose_m(const double P[12], double qr16[16]);
+
+/* ProjectionMatrix::sourcePosition (modules/src/mat/projectionmatrix.cpp:264-274). */
+void oracle_source_position(const double P[12], double c[3]);
+
+/* Ray direction for detector pixel (x, y) from the packed QR, in FP64, before the FP32 cast
+ * (raycasterprojector_interp.cl:74-86 without the final normalize). */
+void oracle_direction64(const double qr16[16], double x, double y, double d[3]);
+
+/*
+ * Forward projection of a float volume (x fastest: index = z*X*Y + y*X + x,
+ * voxelvolume.tpp:310-332) for all configured views, or for a subset of them.
+ *
+ *   pmats    [n_views][n_modules][3][4] row-major FP64 projection matrices
+ *   out      [n_out][n_modules][rows][channels] float, n_out = n_subset or n_views
+ *   out64    optional FP64 copy of the same values (ref64 only; may be NULL)
+ *   fp64     0 -> ref32, 1 -> ref64
+ *   n_samples  optional: total number of sampling-loop iterations (reference loop count)
+ * Returns 0 on success, nonzero on invalid arguments.
+ */
+int oracle_project(const float* volume, const uint32_t dims[3], const float voxel_mm[3],
+                   const float offset_mm[3], const double* pmats, uint32_t n_views,
+                   uint32_t n_modules, uint32_t channels, uint32_t rows,
+                   const oracle_settings* settings, int fp64, int n_threads,
+                   const uint32_t* view_subset, uint32_t n_subset, float* out, double* out64,
+                   uint64_t* n_samples);
+
+/* ref32 single-sample helpers, exposed so tests can pin them against brute force. */
+float oracle_sample_linear32(const float* vol, const uint32_t dims[3], float px, float py, float pz);
+double oracle_sample_linear64(const float* vol, const uint32_t dims[3], double px, double py,
+                              double pz);
+/* ref32 slab clip (raycasterprojector_interp.cl:88-148); writes (lambda_min, lambda_max). */
+void oracle_clip32(const float source[3], const float direction[3], const float vol_size[3],
+                   const float vol_corner[3], int interpolate, float minmax[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,162 @@
+"""ctypes binding of libctlproj.so (the C ABI declared in include/ctlproj.h).
+
+The library is loaded from ``ctl_amd/lib/libctlproj.so``; a missing library raises instead of
+falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libctlproj.so"
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_NO_DEVICE = 2
+ERR_DEVICE = 3
+ERR_OUT_OF_MEMORY = 4
+ERR_NOT_CONFIGURED = 5
+ERR_NO_DATA = 6
+ERR_OVER_BUDGET = 7
+ERR_INJECTED = 8
+
+F32, F16, BF16 = 0, 1, 2
+CARM, TUBULAR = 0, 1
+SHORT_SCAN, WOBBLE, CIRCLE_PLUS_LINE, HELICAL, AXIAL = 0, 1, 2, 3, 4
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("rays_per_pixel", C.c_uint32 * 2),
+        ("ray_sampling", C.c_float),
+        ("volume_upsampling", C.c_uint32),
+        ("interpolate", C.c_int32),
+        ("tile_w", C.c_uint32),
+        ("tile_h", C.c_uint32),
+        ("unroll", C.c_uint32),
+    ]
+
+
+class Detector(C.Structure):
+    _fields_ = [
+        ("pixels", C.c_uint32 * 2),
+        ("pixel_mm", C.c_double * 2),
+        ("modules", C.c_uint32),
+        ("cylindrical", C.c_int32),
+        ("angulation", C.c_double),
+        ("module_spacing", C.c_double),
+        ("skew", C.c_double),
+    ]
+
+
+class Gantry(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("carm_span", C.c_double),
+        ("sdd", C.c_double),
+        ("sid", C.c_double),
+        ("rotation", C.c_double),
+        ("pitch", C.c_double),
+        ("tilt", C.c_double),
+    ]
+
+
+class Trajectory(C.Structure):
+    _fields_ = [("type", C.c_int32), ("p", C.c_double * 6)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_int, C.c_void_p)
+
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+_u32p = C.POINTER(C.c_uint32)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_u64p = C.POINTER(C.c_uint64)
+_i = C.c_int
+_u32 = C.c_uint32
+_sz = C.c_size_t
+
+# name -> (restype, argtypes); every symbol of include/ctlproj.h (and the batch/server headers).
+SIGNATURES: dict[str, tuple] = {
+    "ctlproj_settings_default": (None, [C.POINTER(Settings)]),
+    "ctlproj_last_error": (C.c_char_p, []),
+    "ctlproj_device_count": (_i, [C.POINTER(_i)]),
+    "ctlproj_context_create": (_i, [_i, _pp]),
+    "ctlproj_context_destroy": (_i, [_vp]),
+    "ctlproj_volume_create": (_i, [_vp, _f32p, _u32p, _f32p, _f32p, _i, _pp]),
+    "ctlproj_volume_create_from_device": (_i, [_vp, _vp, _u32p, _f32p, _f32p, _i, _pp]),
+    "ctlproj_volume_destroy": (_i, [_vp]),
+    "ctlproj_volume_bytes": (_i, [_vp, C.POINTER(_sz)]),
+    "ctlproj_projector_create": (_i, [_vp, C.POINTER(Settings), _pp]),
+    "ctlproj_projector_destroy": (_i, [_vp]),
+    "ctlproj_projector_set_settings": (_i, [_vp, C.POINTER(Settings)]),
+    "ctlproj_projector_get_settings": (_i, [_vp, C.POINTER(Settings)]),
+    "ctlproj_projector_configure": (_i, [_vp, _f64p, _u32, _u32, _u32, _u32]),
+    "ctlproj_project": (_i, [_vp, _vp, _f32p, _sz, PROGRESS_FN, _vp]),
+    "ctlproj_project_host_volume": (_i, [_vp, _f32p, _u32p, _f32p, _f32p, _f32p, _sz]),
+    "ctlproj_project_device": (_i, [_vp, _vp, _vp, _sz]),
+    "ctlproj_projector_synchronize": (_i, [_vp]),
+    "ctlproj_projector_last_kernel_ms": (_i, [_vp, _f32p]),
+    "ctlproj_projector_count_samples": (_i, [_vp, _vp, _u64p]),
+    "ctlproj_encode_trajectory": (_i, [C.POINTER(Gantry), C.POINTER(Detector), C.POINTER(Trajectory),
+                                       _u32, _f64p, _sz]),
+    "ctlproj_encode_carm_poses": (_i, [C.c_double, C.POINTER(Detector), _f64p, _f64p, _u32, _f64p, _sz]),
+    "ctlproj_phantom_ellipsoids": (_i, [_u32p, _f32p, _u32, C.c_uint64, C.c_float, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, _f32p]),
+    "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
+}
+
+_lib: C.CDLL | None = None
+
+
+class CtlProjError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+        self.message = message
+
+
+def lib() -> C.CDLL:
+    """Loads libctlproj.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m ctl_amd.build` first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().ctlproj_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise CtlProjError(status, last_error())
+
+
+def fptr(a) -> C.POINTER(C.c_float):
+    return a.ctypes.data_as(_f32p)
+
+
+def dptr(a) -> C.POINTER(C.c_double):
+    return a.ctypes.data_as(_f64p)
+
+
+def u32arr(*v) -> C.Array:
+    return (C.c_uint32 * len(v))(*v)
+
+
+def f32arr(*v) -> C.Array:
+    return (C.c_float * len(v))(*v)

@@ -260,7 +260,7 @@ ctlproj_status prepareGeometry(ctlproj_projector* p, const EffectiveVolume& ev)
 }
 
 ctlproj_status checkProject(ctlproj_projector* p, const ctlproj_volume* vol, const void* out,
-                            size_t out_count)
+                            size_t out_count, bool need_out = true)
 {
     if (!p) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "projector is NULL");
     if (!p->configured) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "project() called before configure()");
@@ -272,7 +272,7 @@ ctlproj_status checkProject(ctlproj_projector* p, const ctlproj_volume* vol, con
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT,
                     "output holds " + std::to_string(out_count) + " values, projection needs " +
                         std::to_string(need));
-    if (need && !out) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output pointer is NULL");
+    if (need_out && need && !out) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output pointer is NULL");
     if (injected(p->ctx)) return CTLPROJ_ERR_INJECTED;
     return CTLPROJ_OK;
 }
@@ -642,7 +642,8 @@ ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlpr
                                                uint64_t* n_samples)
 {
     if (!n_samples) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "n_samples is NULL");
-    ctlproj_status st = checkProject(p, vol, nullptr, size_t(p ? p->V : 0) * (p ? p->M * p->R * p->C : 0));
+    ctlproj_status st =
+        checkProject(p, vol, nullptr, p ? size_t(p->V) * p->M * p->R * p->C : 0, /*need_out=*/false);
     if (st != CTLPROJ_OK) return st;
     *n_samples = 0;
     if (p->V == 0) return CTLPROJ_OK;

@@ -1,0 +1,442 @@
+"""Python mirror of CTL's projector interface over the C ABI.
+
+Names follow the reference: ``RayCasterProjector`` with ``settings()``, ``configure(setup)`` and
+``project(volume)`` (modules/src/projectors/raycasterprojector.h:20-58), ``VoxelVolume`` /
+``VolumeData`` (img/voxelvolume.h), ``ProjectionData`` (img/projectiondata.h),
+``AcquisitionSetup`` with systems and trajectory protocols (acquisition/), and
+``GeometryEncoder.encodeFullGeometry`` (acquisition/geometryencoder.cpp:83-98).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+DEG = np.pi / 180.0
+
+# ----------------------------------------------------------------------------------------------
+# containers
+
+
+class VoxelVolume:
+    """Float volume, x fastest (voxelvolume.tpp:310-332). ``data`` has shape (Z, Y, X)."""
+
+    def __init__(self, data, voxel_size=(1.0, 1.0, 1.0), offset=(0.0, 0.0, 0.0)):
+        self.data = np.ascontiguousarray(data, dtype=np.float32)
+        if self.data.ndim != 3:
+            raise ValueError("volume data must be 3-D (Z, Y, X)")
+        self.voxel_size = tuple(float(v) for v in voxel_size)
+        self.offset = tuple(float(v) for v in offset)
+
+    @property
+    def dims(self) -> tuple[int, int, int]:
+        z, y, x = self.data.shape
+        return (x, y, z)
+
+    def hasData(self) -> bool:
+        return self.data.size > 0
+
+    def smallestVoxelSize(self) -> float:
+        return min(self.voxel_size)
+
+    @staticmethod
+    def cube(n: int, voxel_size: float, value: float) -> "VoxelVolume":
+        return VoxelVolume(np.full((n, n, n), value, np.float32), (voxel_size,) * 3)
+
+
+VolumeData = VoxelVolume
+
+
+class ProjectionData:
+    """Projections as an array (views, modules, rows, channels) (projectiondata.cpp:134-228)."""
+
+    def __init__(self, data: np.ndarray):
+        self.data = data
+
+    def nbViews(self) -> int:
+        return self.data.shape[0]
+
+    def view(self, v: int) -> np.ndarray:
+        return self.data[v]
+
+    def toVector(self) -> np.ndarray:
+        return self.data.reshape(-1)
+
+    @property
+    def dimensions(self):
+        v, m, r, c = self.data.shape
+        return dict(nbChannels=c, nbRows=r, nbModules=m, nbViews=v)
+
+
+# ----------------------------------------------------------------------------------------------
+# system components and protocols
+
+
+@dataclass
+class FlatPanelDetector:
+    nbPixels: tuple[int, int]  # (channels, rows)
+    pixelDimensions: tuple[float, float]  # mm
+
+    def _c(self) -> _lib.Detector:
+        d = _lib.Detector()
+        d.pixels[:] = self.nbPixels
+        d.pixel_mm[:] = self.pixelDimensions
+        d.modules = 1
+        d.cylindrical = 0
+        return d
+
+
+@dataclass
+class CylindricalDetector:
+    nbPixelPerModule: tuple[int, int]
+    pixelDimensions: tuple[float, float]
+    nbDetectorModules: int
+    angulationPerModule: float  # rad
+    moduleSpacing: float = 0.0
+
+    def _c(self) -> _lib.Detector:
+        d = _lib.Detector()
+        d.pixels[:] = self.nbPixelPerModule
+        d.pixel_mm[:] = self.pixelDimensions
+        d.modules = self.nbDetectorModules
+        d.cylindrical = 1
+        d.angulation = self.angulationPerModule
+        d.module_spacing = self.moduleSpacing
+        return d
+
+
+@dataclass
+class CarmGantry:
+    cArmSpan: float = 1000.0
+
+
+@dataclass
+class TubularGantry:
+    sourceToDetectorDistance: float
+    sourceToIsoCenterDistance: float
+    rotationAngle: float = 0.0
+    pitchPosition: float = 0.0
+    tiltAngle: float = 0.0
+
+
+def _gantry(g) -> _lib.Gantry:
+    out = _lib.Gantry()
+    if isinstance(g, CarmGantry):
+        out.type = _lib.CARM
+        out.carm_span = g.cArmSpan
+    else:
+        out.type = _lib.TUBULAR
+        out.sdd = g.sourceToDetectorDistance
+        out.sid = g.sourceToIsoCenterDistance
+        out.rotation = g.rotationAngle
+        out.pitch = g.pitchPosition
+        out.tilt = g.tiltAngle
+    return out
+
+
+@dataclass
+class CTSystem:
+    gantry: object
+    detector: object
+
+
+def _traj(kind: int, *p: float) -> _lib.Trajectory:
+    t = _lib.Trajectory()
+    t.type = kind
+    for i, v in enumerate(p):
+        t.p[i] = v
+    return t
+
+
+class ShortScanTrajectory:  # trajectories.cpp:177-205
+    def __init__(self, sourceToIsocenter: float, startAngle: float = 0.0, angleSpan: float = -1.0):
+        self.c = _traj(_lib.SHORT_SCAN, sourceToIsocenter, startAngle, angleSpan)
+
+
+class WobbleTrajectory:  # trajectories.cpp:71-96
+    def __init__(self, angleSpan, sourceToIsocenter, startAngle=0.0, wobbleAngle=15.0 * DEG,
+                 wobbleFreq=1.0):
+        self.c = _traj(_lib.WOBBLE, angleSpan, sourceToIsocenter, startAngle, wobbleAngle, wobbleFreq)
+
+
+class CirclePlusLineTrajectory:  # trajectories.cpp:125-175
+    def __init__(self, angleSpan, sourceToIsocenter, lineLength, fractionOfViewsForLine,
+                 startAngle=0.0):
+        self.c = _traj(_lib.CIRCLE_PLUS_LINE, angleSpan, sourceToIsocenter, lineLength,
+                       fractionOfViewsForLine, startAngle)
+
+
+class HelicalTrajectory:  # trajectories.cpp:33-50
+    def __init__(self, angleIncrement, pitchIncrement=0.0, startPitch=0.0, startAngle=0.0):
+        self.c = _traj(_lib.HELICAL, angleIncrement, pitchIncrement, startPitch, startAngle)
+
+
+class AxialScanTrajectory:  # trajectories.cpp:239-256
+    def __init__(self, startAngle=0.0):
+        self.c = _traj(_lib.AXIAL, startAngle)
+
+
+class AcquisitionSetup:
+    """System + number of views + preparation protocol (acquisitionsetup.h:182-245)."""
+
+    def __init__(self, system: CTSystem, nbViews: int = 0):
+        self.system = system
+        self._nb_views = int(nbViews)
+        self._protocol = None
+        self._poses = None
+
+    def setNbViews(self, n: int) -> None:
+        self._nb_views = int(n)
+
+    def nbViews(self) -> int:
+        return self._nb_views
+
+    def applyPreparationProtocol(self, protocol) -> None:
+        self._protocol = protocol
+        self._poses = None
+
+    def setCarmPoses(self, positions, rotations) -> None:
+        """Per-view C-arm source locations (prepare::CarmGantryParam::setLocation)."""
+        self._poses = (np.ascontiguousarray(positions, np.float64),
+                       np.ascontiguousarray(rotations, np.float64))
+        self._nb_views = len(self._poses[0])
+        self._protocol = None
+
+
+class GeometryEncoder:
+    @staticmethod
+    def encodeFullGeometry(setup: AcquisitionSetup) -> np.ndarray:
+        """FP64 projection matrices, shape (views, modules, 3, 4) (geometryencoder.cpp:83-98)."""
+        det = setup.system.detector._c()
+        modules = det.modules if det.cylindrical else 1
+        V = setup.nbViews()
+        out = np.zeros((V, modules, 3, 4), np.float64)
+        if setup._poses is not None:
+            pos, rot = setup._poses
+            check(lib().ctlproj_encode_carm_poses(setup.system.gantry.cArmSpan, C.byref(det),
+                                                  _lib.dptr(pos), _lib.dptr(rot), V,
+                                                  _lib.dptr(out), out.size))
+        else:
+            if setup._protocol is None:
+                raise ValueError("no preparation protocol applied")
+            g = _gantry(setup.system.gantry)
+            check(lib().ctlproj_encode_trajectory(C.byref(g), C.byref(det), C.byref(setup._protocol.c),
+                                                  V, _lib.dptr(out), out.size))
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
+# environment and projector
+
+
+class Context:
+    """A device context (replaces OpenCLConfig); one per process and device is enough."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        check(lib().ctlproj_context_create(device, C.byref(self.h)))
+
+    def close(self) -> None:
+        if self.h:
+            lib().ctlproj_context_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+STORAGE = {"f32": _lib.F32, "f16": _lib.F16, "bf16": _lib.BF16}
+
+
+class ResidentVolume:
+    """A volume kept in HBM (FP32, FP16 or BF16 storage) across projections."""
+
+    def __init__(self, volume: VoxelVolume, storage: str = "f32", ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.h = C.c_void_p()
+        self.volume = volume
+        self.storage = storage
+        check(lib().ctlproj_volume_create(
+            self.ctx.h, _lib.fptr(volume.data), _lib.u32arr(*volume.dims),
+            _lib.f32arr(*volume.voxel_size), _lib.f32arr(*volume.offset), STORAGE[storage],
+            C.byref(self.h)))
+
+    @classmethod
+    def from_device(cls, ptr: int, dims, voxel_size, offset=(0.0, 0.0, 0.0), storage="f32",
+                    ctx: Context | None = None) -> "ResidentVolume":
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        self.h = C.c_void_p()
+        self.volume = None
+        self.storage = storage
+        check(lib().ctlproj_volume_create_from_device(
+            self.ctx.h, C.c_void_p(ptr), _lib.u32arr(*dims), _lib.f32arr(*voxel_size),
+            _lib.f32arr(*offset), STORAGE[storage], C.byref(self.h)))
+        return self
+
+    def nbytes(self) -> int:
+        n = C.c_size_t()
+        check(lib().ctlproj_volume_bytes(self.h, C.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().ctlproj_volume_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Settings:
+    """RayCasterProjector::Settings (raycasterprojector.h:27-37) plus tuning fields."""
+    raysPerPixel: list = field(default_factory=lambda: [1, 1])
+    raySampling: float = 0.3
+    volumeUpSampling: int = 1
+    interpolate: bool = True
+    tile: tuple | None = None  # (wave_w, wave_h), 0 = automatic
+    unroll: int = 0
+
+    def _c(self) -> _lib.Settings:
+        s = _lib.Settings()
+        s.rays_per_pixel[:] = self.raysPerPixel
+        s.ray_sampling = self.raySampling
+        s.volume_upsampling = self.volumeUpSampling
+        s.interpolate = 1 if self.interpolate else 0
+        if self.tile:
+            s.tile_w, s.tile_h = self.tile
+        s.unroll = self.unroll
+        return s
+
+
+class RayCasterProjector:
+    """Drop-in for CTL::OCL::RayCasterProjector (raycasterprojector.h:20-58)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self._settings = Settings()
+        self.h = C.c_void_p()
+        s = self._settings._c()
+        check(lib().ctlproj_projector_create(self.ctx.h, C.byref(s), C.byref(self.h)))
+        self._dims = None
+        self.progress = None  # callable(view) -- ProjectorNotifier::projectionFinished
+
+    def settings(self) -> Settings:
+        return self._settings
+
+    def _push_settings(self) -> None:
+        s = self._settings._c()
+        check(lib().ctlproj_projector_set_settings(self.h, C.byref(s)))
+
+    def configure(self, setup) -> None:
+        """configure(AcquisitionSetup) or configure((pmats, channels, rows))."""
+        if isinstance(setup, AcquisitionSetup):
+            pm = GeometryEncoder.encodeFullGeometry(setup)
+            det = setup.system.detector
+            px = det.nbPixels if isinstance(det, FlatPanelDetector) else det.nbPixelPerModule
+            self.configure_matrices(pm, px[0], px[1])
+        else:
+            pm, c, r = setup
+            self.configure_matrices(pm, c, r)
+
+    def configure_matrices(self, pmats: np.ndarray, channels: int, rows: int) -> None:
+        pm = np.ascontiguousarray(pmats, np.float64)
+        if pm.ndim == 3:
+            pm = pm[:, None]
+        V, M = pm.shape[:2]
+        check(lib().ctlproj_projector_configure(self.h, _lib.dptr(pm), V, M, channels, rows))
+        self._dims = (V, M, rows, channels)
+        self._pmats = pm
+
+    def _out(self) -> np.ndarray:
+        if self._dims is None:
+            raise _lib.CtlProjError(_lib.ERR_NOT_CONFIGURED, "project() called before configure()")
+        return np.empty(self._dims, np.float32)
+
+    def project(self, volume) -> ProjectionData:
+        """project(VolumeData) -> ProjectionData; `volume` may also be a ResidentVolume."""
+        self._push_settings()
+        out = self._out()
+        if isinstance(volume, ResidentVolume):
+            cb = _lib.PROGRESS_FN(lambda v, u: self.progress(v)) if self.progress else _lib.PROGRESS_FN()
+            check(lib().ctlproj_project(self.h, volume.h, _lib.fptr(out), out.size, cb, None))
+        else:
+            if not volume.hasData():
+                return ProjectionData(np.zeros((0,) + out.shape[1:], np.float32))
+            check(lib().ctlproj_project_host_volume(
+                self.h, _lib.fptr(volume.data), _lib.u32arr(*volume.dims),
+                _lib.f32arr(*volume.voxel_size), _lib.f32arr(*volume.offset), _lib.fptr(out),
+                out.size))
+        return ProjectionData(out)
+
+    def project_device(self, volume: ResidentVolume, out_ptr: int, count: int) -> None:
+        self._push_settings()
+        check(lib().ctlproj_project_device(self.h, volume.h, C.c_void_p(out_ptr), count))
+
+    def synchronize(self) -> None:
+        check(lib().ctlproj_projector_synchronize(self.h))
+
+    def last_kernel_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().ctlproj_projector_last_kernel_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def count_samples(self, volume: ResidentVolume) -> int:
+        self._push_settings()
+        n = C.c_uint64()
+        check(lib().ctlproj_projector_count_samples(self.h, volume.h, C.byref(n)))
+        return n.value
+
+    def isLinear(self) -> bool:  # abstractprojector.h:97
+        return True
+
+    def projectComposite(self, volumes) -> ProjectionData:
+        """Sum of the projections of several volumes (abstractprojector.h:99-112)."""
+        total = None
+        for v in volumes:
+            p = self.project(v).data
+            total = p if total is None else total + p
+        return ProjectionData(total)
+
+    def close(self) -> None:
+        if self.h:
+            lib().ctlproj_projector_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ellipsoid_phantom(dims, voxel_mm, n_ellipsoids, seed, semi_axes=(4.0, 24.0),
+                      mu=(0.005, 0.03), background=0.02) -> np.ndarray:
+    """Seeded random-ellipsoid phantom, shape (Z, Y, X) float32."""
+    x, y, z = dims
+    out = np.empty((z, y, x), np.float32)
+    check(lib().ctlproj_phantom_ellipsoids(_lib.u32arr(*dims), _lib.f32arr(*voxel_mm), n_ellipsoids,
+                                           seed, semi_axes[0], semi_axes[1], mu[0], mu[1],
+                                           background, _lib.fptr(out)))
+    return out

@@ -1,0 +1,91 @@
+"""Benchmark and parity workloads: seeded ellipsoid phantoms in CTL acquisition geometries.
+
+  small    64^3 x 1 mm, 10 ellipsoids; C-arm span 1200 mm, source-isocentre 750 mm, 36 views over
+           360 deg (10 deg apart), flat panel 256x128 @ 1 mm, ray step 0.1 voxel.
+  real     256^3 x 0.5 mm, 20 ellipsoids; same C-arm, 360 views 1 deg apart, flat panel 1024x768
+           @ 0.4 mm, ray step 0.2 voxel.
+  helical  512x512x256 @ 0.7/0.7/1.0 mm, 40 ellipsoids (0-0.04 /mm); tubular gantry SDD 1085 mm,
+           SID 595 mm, 2000 views over 2 turns at pitch 1.0, cylindrical detector of 40 modules of
+           16x64 pixels @ 1 mm, ray step 0.25 voxel.
+A full circle of n views uses ShortScanTrajectory(750, 0, (n-1)/n * 360 deg) because the protocol
+spaces views by span/(n-1) (trajectories.cpp:195). Phantom seeds are 1000*config + instance.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .projector import (DEG, AcquisitionSetup, CarmGantry, CTSystem, CylindricalDetector,
+                        FlatPanelDetector, GeometryEncoder, HelicalTrajectory, ShortScanTrajectory,
+                        TubularGantry, ellipsoid_phantom)
+
+
+@dataclass
+class Workload:
+    name: str
+    volume: np.ndarray  # (Z, Y, X) float32
+    voxel_mm: tuple
+    offset_mm: tuple
+    pmats: np.ndarray   # (V, M, 3, 4)
+    channels: int
+    rows: int
+    ray_sampling: float
+
+    @property
+    def views(self) -> int:
+        return self.pmats.shape[0]
+
+    @property
+    def modules(self) -> int:
+        return self.pmats.shape[1]
+
+    @property
+    def dims(self):
+        z, y, x = self.volume.shape
+        return (x, y, z)
+
+
+def carm_circle(n_views: int, pixels, pitch_mm, span=1200.0, sid=750.0) -> np.ndarray:
+    setup = AcquisitionSetup(CTSystem(CarmGantry(span), FlatPanelDetector(pixels, pitch_mm)), n_views)
+    setup.applyPreparationProtocol(ShortScanTrajectory(sid, 0.0, 360.0 * DEG * (n_views - 1) / n_views))
+    return GeometryEncoder.encodeFullGeometry(setup)
+
+
+def helical_geometry(n_views=2000, turns=2.0, pitch=1.0, sdd=1085.0, sid=595.0,
+                     modules=40, module_px=(16, 64), pixel_mm=1.0) -> np.ndarray:
+    # contiguous flat modules whose centres lie on a circle of radius SDD about the source:
+    # (w/2) * cot(alpha/2) = SDD
+    w = module_px[0] * pixel_mm
+    alpha = 2.0 * math.atan(0.5 * w / sdd)
+    det = CylindricalDetector(module_px, (pixel_mm, pixel_mm), modules, alpha, 0.0)
+    coverage_iso = module_px[1] * pixel_mm * sid / sdd
+    views_per_turn = n_views / turns
+    feed = pitch * coverage_iso / views_per_turn
+    setup = AcquisitionSetup(CTSystem(TubularGantry(sdd, sid), det), n_views)
+    setup.applyPreparationProtocol(HelicalTrajectory(360.0 * DEG / views_per_turn, feed,
+                                                     -0.5 * feed * n_views, 0.0))
+    return GeometryEncoder.encodeFullGeometry(setup)
+
+
+def small(views: int = 36) -> Workload:
+    vol = ellipsoid_phantom((64, 64, 64), (1.0, 1.0, 1.0), 10, 1000)
+    return Workload("small: 64^3 f32, 36 views, 256x128, step 0.1 vox", vol, (1.0, 1.0, 1.0),
+                    (0.0, 0.0, 0.0), carm_circle(views, (256, 128), (1.0, 1.0)), 256, 128, 0.1)
+
+
+def real(views: int = 360, n: int = 256, panel=(1024, 768)) -> Workload:
+    vs = 0.5 * 256 / n
+    vol = ellipsoid_phantom((n, n, n), (vs, vs, vs), 20, 2000)
+    pitch = 0.4 * 1024 / panel[0]
+    return Workload(f"real: {n}^3, {views} views, {panel[0]}x{panel[1]}, step 0.2 vox", vol,
+                    (vs, vs, vs), (0.0, 0.0, 0.0), carm_circle(views, panel, (pitch, pitch)),
+                    panel[0], panel[1], 0.2)
+
+
+def helical(views: int = 2000, dims=(512, 512, 256)) -> Workload:
+    vox = (0.7 * 512 / dims[0], 0.7 * 512 / dims[1], 1.0 * 256 / dims[2])
+    vol = ellipsoid_phantom(dims, vox, 40, 3000, mu=(0.0, 0.04))
+    return Workload(f"helical: {dims[0]}x{dims[1]}x{dims[2]}, {views} views, 40x16x64, step 0.25 vox",
+                    vol, vox, (0.0, 0.0, 0.0), helical_geometry(views), 16, 64, 0.25)

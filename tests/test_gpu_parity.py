@@ -1,0 +1,212 @@
+"""Parity of the HIP projector (through the C ABI) with the CPU oracle.
+
+Bars (fixed here, not tuned per run):
+  FP32 storage vs ref32:  relative L2 <= 1e-4 and max over pixels of |d| / max(|ref|, 1e-3*max|ref|)
+                          <= 1e-3 (errormetrics.cpp:69-85 for the L2 definition, reference first);
+  FP16 storage vs ref64:  relative L2 <= 5e-4;
+  BF16 storage vs ref64:  relative L2 <= 4e-3 (8-bit mantissa; reported, not a drop-in claim).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from ctl_amd import (RayCasterProjector, ResidentVolume, VoxelVolume, CtlProjError)
+from ctl_amd import _lib
+from ctl_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+L2_F32, MAR_F32, L2_F16, L2_BF16 = 1e-4, 1e-3, 5e-4, 4e-3
+
+
+def run_gpu(w, storage="f32", pm=None, **settings):
+    pm = w.pmats if pm is None else pm
+    p = RayCasterProjector()
+    s = p.settings()
+    s.raySampling = w.ray_sampling
+    for k, v in settings.items():
+        setattr(s, k, v)
+    p.configure((pm, w.channels, w.rows))
+    vol = ResidentVolume(VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
+    out = p.project(vol).data
+    return out, p, vol
+
+
+def run_ref(w, pm=None, fp64=False, **kw):
+    pm = w.pmats if pm is None else pm
+    return oracle.project(w.volume, w.voxel_mm, w.offset_mm, pm, w.channels, w.rows,
+                          w.ray_sampling, fp64=fp64, **kw)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return W.small()
+
+
+@pytest.fixture(scope="module")
+def small_refs(small):
+    r32, n32 = run_ref(small)
+    r64, n64 = run_ref(small, fp64=True)
+    return r32, n32, r64
+
+
+def test_small_fp32_matches_ref32(small, small_refs):
+    r32, _, r64 = small_refs
+    got, _, _ = run_gpu(small)
+    assert got.shape == (36, 1, 128, 256)
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"small f32: rel L2 vs ref32 {e:.3e}, max abs rel {m:.3e}, vs ref64 {oracle.rel_l2(r64, got):.3e}")
+    assert e <= L2_F32 and m <= MAR_F32
+
+
+@pytest.mark.parametrize("storage,bar", [("f16", L2_F16), ("bf16", L2_BF16)])
+def test_small_half_storage_vs_ref64(small, small_refs, storage, bar):
+    _, _, r64 = small_refs
+    got, _, _ = run_gpu(small, storage)
+    e = oracle.rel_l2(r64, got)
+    print(f"small {storage}: rel L2 vs ref64 {e:.3e}")
+    assert e <= bar
+
+
+def test_sample_count_matches_oracle(small, small_refs):
+    _, n32, _ = small_refs
+    p = RayCasterProjector()
+    p.settings().raySampling = small.ray_sampling
+    p.configure((small.pmats, small.channels, small.rows))
+    vol = ResidentVolume(VoxelVolume(small.volume, small.voxel_mm, small.offset_mm))
+    n = p.count_samples(vol)
+    print(f"samples: gpu {n} oracle {n32}")
+    assert abs(n - n32) <= 1e-5 * n32
+
+
+def test_unroll_and_tile_variants_agree(small, small_refs):
+    r32 = small_refs[0]
+    for tile in [(8, 8), (16, 4), (4, 16), (32, 2), (64, 1)]:
+        for unroll in (1, 2, 4):
+            got, _, _ = run_gpu(small, pm=small.pmats[::6], tile=tile, unroll=unroll)
+            e = oracle.rel_l2(r32[::6], got)
+            assert e <= L2_F32, (tile, unroll, e)
+
+
+def test_nearest_mode(small):
+    pm = small.pmats[::4]
+    r32, _ = run_ref(small, pm=pm, interpolate=False)
+    got, _, _ = run_gpu(small, pm=pm, interpolate=False)
+    e = oracle.rel_l2(r32, got)
+    print(f"nearest: rel L2 {e:.3e}")
+    assert e <= L2_F32
+
+
+def test_rays_per_pixel(small):
+    pm = small.pmats[::9]
+    r32, _ = run_ref(small, pm=pm, rays_per_pixel=(2, 3))
+    got, _, _ = run_gpu(small, pm=pm, raysPerPixel=[2, 3])
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    assert e <= L2_F32 and m <= MAR_F32
+
+
+@pytest.mark.parametrize("factor", [0, 2])
+def test_volume_upsampling(small, factor):
+    pm = small.pmats[::9]
+    r32, _ = run_ref(small, pm=pm, upsampling=factor)
+    got, _, _ = run_gpu(small, pm=pm, volumeUpSampling=factor)
+    e = oracle.rel_l2(r32, got)
+    print(f"upsampling {factor}: rel L2 {e:.3e}")
+    assert e <= L2_F32
+
+
+def test_helical_multimodule():
+    pm = W.helical_geometry(n_views=12, turns=0.05)
+    vol = np.ascontiguousarray(W.ellipsoid_phantom((128, 128, 64), (2.8, 2.8, 4.0), 12, 3001,
+                                                   mu=(0.0, 0.04)))
+    w = W.Workload("helical small", vol, (2.8, 2.8, 4.0), (0.0, 0.0, 0.0), pm, 16, 64, 0.25)
+    r32, _ = run_ref(w)
+    got, _, _ = run_gpu(w)
+    assert got.shape == (12, 40, 64, 16)
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"helical 40 modules: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= L2_F32 and m <= MAR_F32
+
+
+def test_real_size_views_against_oracle():
+    w = W.real(views=360)
+    pick = [0, 137]
+    r32, _ = run_ref(w, views=pick)
+    got, _, _ = run_gpu(w, pm=w.pmats[pick])
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"256^3 1024x768 views {pick}: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= L2_F32 and m <= MAR_F32
+
+
+def test_real_size_linearity_and_device_path():
+    """At the full 360-view size: project(a*V1 + b*V2) = a*P1 + b*P2 and the device-output path
+    equals the host-output path bit for bit."""
+    import ctypes
+    w = W.real(views=360)
+    v2 = np.ascontiguousarray(w.volume[::-1, :, ::-1])
+    p = RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((w.pmats, w.channels, w.rows))
+    P1 = p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm))).data
+    P2 = p.project(ResidentVolume(VoxelVolume(v2, w.voxel_mm))).data
+    P12 = p.project(ResidentVolume(VoxelVolume(2.0 * w.volume + 0.5 * v2, w.voxel_mm))).data
+    e = oracle.rel_l2(2.0 * P1.astype(np.float64) + 0.5 * P2, P12)
+    print(f"linearity rel L2 {e:.3e}")
+    assert e <= 1e-5
+    import torch
+    vol = ResidentVolume(VoxelVolume(w.volume, w.voxel_mm))
+    dev = torch.empty(P1.size, dtype=torch.float32, device="cuda")
+    p.project_device(vol, dev.data_ptr(), dev.numel())
+    p.synchronize()
+    assert np.array_equal(dev.cpu().numpy().reshape(P1.shape), P1)
+
+
+def test_zero_volume_and_miss():
+    w = W.small(views=4)
+    zero = W.Workload("zero", np.zeros_like(w.volume), w.voxel_mm, w.offset_mm, w.pmats, 256, 128, 0.1)
+    got, _, _ = run_gpu(zero)
+    assert np.all(got == 0.0)
+    # volume shifted far outside every ray: all rays miss
+    far = W.Workload("far", w.volume, w.voxel_mm, (0.0, 0.0, 5000.0), w.pmats, 256, 128, 0.1)
+    got, _, _ = run_gpu(far)
+    assert np.all(got == 0.0)
+
+
+def test_error_paths():
+    w = W.small(views=4)
+    p = RayCasterProjector()
+    with pytest.raises(CtlProjError) as ei:  # project before configure
+        p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm)))
+    assert ei.value.status == _lib.ERR_NOT_CONFIGURED
+    p.configure((w.pmats, w.channels, w.rows))
+    # a volume with no data gives zero views, as the reference does (raycasterprojector.cpp:156-160)
+    empty = p.project(VoxelVolume(np.zeros((0, 0, 0), np.float32)))
+    assert empty.nbViews() == 0
+    with pytest.raises(CtlProjError) as ei:
+        _lib.check(_lib.lib().ctlproj_context_create(99, ctypes_ref()))
+    assert ei.value.status == _lib.ERR_NO_DEVICE
+    # forced device failure: an error status, no partial result reported as success
+    _lib.check(_lib.lib().ctlproj_debug_inject_failure(p.ctx.h, 1))
+    with pytest.raises(CtlProjError) as ei:
+        p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm)))
+    assert ei.value.status == _lib.ERR_INJECTED
+    out = p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm))).data
+    assert np.isfinite(out).all() and out.max() > 0
+
+
+def ctypes_ref():
+    import ctypes
+    return ctypes.byref(ctypes.c_void_p())
+
+
+def test_progress_callback_order():
+    w = W.small(views=36)
+    p = RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((w.pmats, w.channels, w.rows))
+    seen = []
+    p.progress = seen.append
+    p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm)))
+    assert seen == list(range(36))

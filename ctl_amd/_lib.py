@@ -123,6 +123,12 @@ def lib() -> C.CDLL:
     """Loads libctlproj.so once; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch's wheel carries its own libamdhip64.so.7 (same soname as ROCm's): load torch
+        # first so that both use one HIP runtime in this process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: run `python -m ctl_amd.build` first")
         L = C.CDLL(str(LIB_PATH))

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <new>
@@ -86,10 +87,11 @@ ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage, uint32_t, u
 
 namespace {
 // Limits of the gather addressing: 24-bit row/slice multiplies and 32-bit buffer offsets.
-ctlproj_status checkPaddedSize(const uint32_t n[3], uint32_t f, ctlk::Storage st, size_t* bytes)
+ctlproj_status checkPaddedSize(const uint32_t n[3], uint32_t f, ctlk::Storage st, ctlk::Layout ly,
+                               size_t* bytes)
 {
     const uint64_t px = uint64_t(n[0]) * f + 2, py = uint64_t(n[1]) * f + 2, pz = uint64_t(n[2]) * f + 2;
-    const uint64_t eb = ctlk::storageBytes(st);
+    const uint64_t eb = ctlk::texelBytes(st, ly);
     if (px * py * eb >= (1ull << 24))
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume slice too large (X*Y*bytes must be < 16 MiB)");
     const uint64_t b = px * py * pz * eb;
@@ -99,7 +101,7 @@ ctlproj_status checkPaddedSize(const uint32_t n[3], uint32_t f, ctlk::Storage st
     return CTLPROJ_OK;
 }
 
-ctlk::VolumeView makeView(void* d, const uint32_t n[3], ctlk::Storage st, size_t bytes)
+ctlk::VolumeView makeView(void* d, const uint32_t n[3], ctlk::Storage st, ctlk::Layout ly, size_t bytes)
 {
     ctlk::VolumeView v{};
     v.data = d;
@@ -108,6 +110,7 @@ ctlk::VolumeView makeView(void* d, const uint32_t n[3], ctlk::Storage st, size_t
     v.pitch_z = (n[0] + 2) * (n[1] + 2);
     v.bytes = uint32_t(bytes);
     v.storage = st;
+    v.layout = ly;
     return v;
 }
 } // namespace
@@ -128,7 +131,7 @@ ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings
     } else if (f == 0) { // case 0: one voxel holding the mean (:592-610)
         const uint32_t one[3] = { 1, 1, 1 };
         size_t bytes = 0;
-        ctlproj_status st = checkPaddedSize(one, 1, vol->view.storage, &bytes);
+        ctlproj_status st = checkPaddedSize(one, 1, vol->view.storage, vol->view.layout, &bytes);
         if (st != CTLPROJ_OK) return st;
         const size_t sum_at = (bytes + 7) / 8 * 8;
         hipError_t e = hipMalloc(&ev.tmp, sum_at + sizeof(double));
@@ -138,16 +141,17 @@ ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings
             return hipFail(e, "hipMemsetAsync");
         if ((e = ctlk::launchSum(vol->view, d_sum, stream)) != hipSuccess) return hipFail(e, "sum kernel");
         const double count = double(vol->n[0]) * vol->n[1] * vol->n[2];
-        if ((e = ctlk::launchFillMean(d_sum, count, vol->view.storage, ev.tmp, stream)) != hipSuccess)
+        if ((e = ctlk::launchFillMean(d_sum, count, vol->view.storage, vol->view.layout, ev.tmp,
+                                      stream)) != hipSuccess)
             return hipFail(e, "fill kernel");
-        ev.view = makeView(ev.tmp, one, vol->view.storage, bytes);
+        ev.view = makeView(ev.tmp, one, vol->view.storage, vol->view.layout, bytes);
         for (int k = 0; k < 3; ++k) {
             g.n[k] = 1;
             g.voxel_mm[k] = vol->voxel_mm[k] * float(vol->n[k]);
         }
     } else { // case f > 1: replicate each voxel f^3 times (:625-659), on the device
         size_t bytes = 0;
-        ctlproj_status st = checkPaddedSize(vol->n, f, vol->view.storage, &bytes);
+        ctlproj_status st = checkPaddedSize(vol->n, f, vol->view.storage, vol->view.layout, &bytes);
         if (st != CTLPROJ_OK) return st;
         hipError_t e = hipMalloc(&ev.tmp, bytes);
         if (e != hipSuccess) return hipFail(e, "hipMalloc(upsampled volume)");
@@ -159,7 +163,7 @@ ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings
             g.n[k] = nn[k];
             g.voxel_mm[k] = vol->voxel_mm[k] / float(f);
         }
-        ev.view = makeView(ev.tmp, nn, vol->view.storage, bytes);
+        ev.view = makeView(ev.tmp, nn, vol->view.storage, vol->view.layout, bytes);
     }
     // increment_mm = min(voxel size) * raySampling (raycasterprojector.cpp:178-179)
     const float smallest = std::min(std::min(g.voxel_mm[0], g.voxel_mm[1]), g.voxel_mm[2]);
@@ -172,8 +176,12 @@ ctlproj_status createVolume(ctlproj_context* ctx, const float* d_src, const uint
                             const float voxel_mm[3], const float offset_mm[3], ctlk::Storage st,
                             ctlproj_volume** out)
 {
+    // Interleaved texels by default; CTLPROJ_VOLUME_LAYOUT=plain selects one voxel per texel.
+    const char* env = std::getenv("CTLPROJ_VOLUME_LAYOUT");
+    const ctlk::Layout ly = (env && std::strcmp(env, "plain") == 0) ? ctlk::Layout::Plain
+                                                                    : ctlk::Layout::Interleaved;
     size_t bytes = 0;
-    ctlproj_status s = checkPaddedSize(dims, 1, st, &bytes);
+    ctlproj_status s = checkPaddedSize(dims, 1, st, ly, &bytes);
     if (s != CTLPROJ_OK) return s;
     ctlproj_volume* v = new (std::nothrow) ctlproj_volume;
     if (!v) return fail(CTLPROJ_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -184,7 +192,7 @@ ctlproj_status createVolume(ctlproj_context* ctx, const float* d_src, const uint
         return hipFail(e, "hipMalloc(volume)");
     }
     v->bytes = bytes;
-    if ((e = ctlk::launchPack(d_src, dims, 1, st, v->d_data, ctx->stream)) != hipSuccess ||
+    if ((e = ctlk::launchPack(d_src, dims, 1, st, ly, v->d_data, ctx->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
         (void)hipFree(v->d_data);
         delete v;
@@ -195,7 +203,7 @@ ctlproj_status createVolume(ctlproj_context* ctx, const float* d_src, const uint
         v->voxel_mm[k] = voxel_mm[k];
         v->offset_mm[k] = offset_mm[k];
     }
-    v->view = makeView(v->d_data, dims, st, bytes);
+    v->view = makeView(v->d_data, dims, st, ly, bytes);
     *out = v;
     return CTLPROJ_OK;
 }

@@ -11,6 +11,17 @@ enum class Storage : int { F32 = 0, F16 = 1, BF16 = 2 };
 
 inline size_t storageBytes(Storage s) { return s == Storage::F32 ? 4 : 2; }
 
+// Texel layouts of a resident volume.
+//   Plain:       one voxel per texel (4 B FP32, 2 B FP16/BF16).
+//   Interleaved: 8-byte texels that carry the neighbours a trilinear sample needs, so one 16-byte
+//                load fetches the taps of two x-adjacent texels:
+//                  FP32       texel(x,y,z) = [v(x,y,z), v(x,y,z+1)]           (2 loads/sample)
+//                  FP16/BF16  texel(x,y,z) = [v(x,y,z), v(x,y+1,z), v(x,y,z+1), v(x,y+1,z+1)]
+//                                                                              (1 load/sample)
+enum class Layout : int { Plain = 0, Interleaved = 1 };
+
+inline size_t texelBytes(Storage s, Layout l) { return l == Layout::Interleaved ? 8 : storageBytes(s); }
+
 // A resident volume in HBM: (X+2) x (Y+2) x (Z+2) texels, x fastest, with a one-texel zero
 // border on every side, so that every tap of a sample inside the support is an in-bounds read.
 struct VolumeView {
@@ -20,6 +31,7 @@ struct VolumeView {
     uint32_t pitch_z;   // (X+2)*(Y+2)  (texels per slice)
     uint32_t bytes;     // total bytes of the padded array (buffer range for the gather loads)
     Storage storage;
+    Layout layout;
 };
 
 // Volume geometry after upsampling, as the host passes it (FP32, raycasterprojector.cpp:169-179).
@@ -47,7 +59,7 @@ static_assert(sizeof(RayGeom) == 64, "RayGeom is one 64-byte record");
 struct LaunchShape {
     uint32_t wave_w = 8, wave_h = 8; // pixel tile per wavefront (wave_w * wave_h = 64)
     uint32_t block_waves_x = 2, block_waves_y = 2; // waves per workgroup
-    uint32_t unroll = 2;
+    uint32_t unroll = 4;
 };
 
 struct RayParams {
@@ -70,15 +82,15 @@ hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayG
 hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
                        unsigned long long* d_count, hipStream_t s);
 
-// float (X,Y,Z) -> padded storage, replicating each voxel f^3 times (f >= 1).
-hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, void* d_dst,
-                      hipStream_t s);
-// padded storage -> padded storage upsampled by f (f >= 1).
+// float (X,Y,Z) -> padded storage in `layout`, replicating each voxel f^3 times (f >= 1).
+hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, Layout layout,
+                      void* d_dst, hipStream_t s);
+// padded storage -> padded storage upsampled by f (f >= 1), same storage and layout.
 hipError_t launchUpsample(const VolumeView& src, uint32_t f, void* d_dst, hipStream_t s);
 // Mean of the interior of a padded volume, into *d_sum (double, accumulated).
 hipError_t launchSum(const VolumeView& src, double* d_sum, hipStream_t s);
 // Fill a one-texel padded volume (1x1x1 interior) with the value *d_sum / count.
-hipError_t launchFillMean(const double* d_sum, double count, Storage st, void* d_dst,
+hipError_t launchFillMean(const double* d_sum, double count, Storage st, Layout layout, void* d_dst,
                           hipStream_t s);
 // Zero-fill of a float array with a plain kernel (graph-capture safe).
 hipError_t launchZero(float* d, size_t n, hipStream_t s);

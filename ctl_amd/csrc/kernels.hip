@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 #include <cfloat>
+#include <climits>
 #include <cstdint>
 
 namespace ctlk {
@@ -201,48 +202,109 @@ __device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, con
 }
 
 // ---------------------------------------------------------------------------------------------
-// trilinear sample at texel-frame position q (padded frame, 0 <= q < n+1): x-adjacent taps are
-// one 8-byte (FP32) or 4-byte (FP16/BF16) load; four loads per sample.
+// trilinear / nearest sample at texel-frame position q (padded frame, blo <= q < n+1)
 
 struct Pitch {
-    uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) rows
+    uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) texel rows
     uint32_t ty, tz;   // texels per row / slice
 };
 
-template <Storage ST> struct Taps;
+__device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
 
-template <> struct Taps<Storage::F32> {
-    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
-    {
-        const auto raw = __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0);
-        const float2 t = __builtin_bit_cast(float2, raw);
-        return fmaf(w, t.y - t.x, t.x);
-    }
-    static constexpr uint32_t kShift = 2;
-};
+__device__ __forceinline__ float halfLo(uint32_t u) { return float(__builtin_bit_cast(half2_t, u).x); }
+__device__ __forceinline__ float halfHi(uint32_t u) { return float(__builtin_bit_cast(half2_t, u).y); }
+__device__ __forceinline__ float bfLo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bfHi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
 
-template <> struct Taps<Storage::F16> {
-    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
-    {
-        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, so, 0);
-        const half2_t h = __builtin_bit_cast(half2_t, raw);
-        const float a = float(h.x), b = float(h.y);
-        return fmaf(w, b, fmaf(-w, a, a));
-    }
-    static constexpr uint32_t kShift = 1;
-};
-
-template <> struct Taps<Storage::BF16> {
-    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
-    {
-        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, so, 0);
-        const float a = __uint_as_float(raw << 16), b = __uint_as_float(raw & 0xffff0000u);
-        return fmaf(w, b - a, a);
-    }
-    static constexpr uint32_t kShift = 1;
-};
-
+// x-lerp of a packed pair (a = element at x, b = element at x+1) in FP32 arithmetic; for FP16 the
+// two FMAs read the halves directly (v_fma_mix).
 template <Storage ST>
+__device__ __forceinline__ float lerpPair(uint32_t lo_word, bool lo_hi, uint32_t hi_word, bool hi_hi, float w)
+{
+    if constexpr (ST == Storage::F16) {
+        const float a = lo_hi ? halfHi(lo_word) : halfLo(lo_word);
+        const float b = hi_hi ? halfHi(hi_word) : halfLo(hi_word);
+        return fmaf(w, b, fmaf(-w, a, a));
+    } else {
+        const float a = lo_hi ? bfHi(lo_word) : bfLo(lo_word);
+        const float b = hi_hi ? bfHi(hi_word) : bfLo(hi_word);
+        return lerp(a, b, w);
+    }
+}
+
+template <Storage ST, Layout LY> struct Sampler;
+
+// FP32, one voxel per texel: four 8-byte loads (x-pairs at y/z corners).
+template <> struct Sampler<Storage::F32, Layout::Plain> {
+    static constexpr uint32_t kShift = 2;
+    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    {
+        const float2 t = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0));
+        return lerp(t.x, t.y, w);
+    }
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const float c00 = lerpx(r, ofs, 0, wx), c10 = lerpx(r, ofs, p.y, wx);
+        const float c01 = lerpx(r, ofs, p.z, wx), c11 = lerpx(r, ofs, p.yz, wx);
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+
+// FP16/BF16, one voxel per texel: four 4-byte loads.
+template <Storage ST> struct PlainHalf {
+    static constexpr uint32_t kShift = 1;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const uint32_t t00 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, 0, 0);
+        const uint32_t t10 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.y, 0);
+        const uint32_t t01 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.z, 0);
+        const uint32_t t11 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.yz, 0);
+        const float c00 = lerpPair<ST>(t00, false, t00, true, wx);
+        const float c10 = lerpPair<ST>(t10, false, t10, true, wx);
+        const float c01 = lerpPair<ST>(t01, false, t01, true, wx);
+        const float c11 = lerpPair<ST>(t11, false, t11, true, wx);
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+template <> struct Sampler<Storage::F16, Layout::Plain> : PlainHalf<Storage::F16> {};
+template <> struct Sampler<Storage::BF16, Layout::Plain> : PlainHalf<Storage::BF16> {};
+
+// FP32 z-pair texels: two 16-byte loads, rows y and y+1, each [v(x,z) v(x,z+1) v(x+1,z) v(x+1,z+1)].
+template <> struct Sampler<Storage::F32, Layout::Interleaved> {
+    static constexpr uint32_t kShift = 3;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+        const float4 b = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, p.y, 0));
+        const float c00 = lerp(a.x, a.z, wx), c01 = lerp(a.y, a.w, wx); // y0: z0, z1
+        const float c10 = lerp(b.x, b.z, wx), c11 = lerp(b.y, b.w, wx); // y1: z0, z1
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+
+// FP16/BF16 (y,z)-quad texels: one 16-byte load = the quads at x and x+1, each
+// [v(y,z) v(y+1,z) | v(y,z+1) v(y+1,z+1)] as two 32-bit words.
+template <Storage ST> struct QuadHalf {
+    static constexpr uint32_t kShift = 3;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch&, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const u4 t = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+        const float c00 = lerpPair<ST>(t.x, false, t.z, false, wx); // y0 z0
+        const float c10 = lerpPair<ST>(t.x, true, t.z, true, wx);   // y1 z0
+        const float c01 = lerpPair<ST>(t.y, false, t.w, false, wx); // y0 z1
+        const float c11 = lerpPair<ST>(t.y, true, t.w, true, wx);   // y1 z1
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+template <> struct Sampler<Storage::F16, Layout::Interleaved> : QuadHalf<Storage::F16> {};
+template <> struct Sampler<Storage::BF16, Layout::Interleaved> : QuadHalf<Storage::BF16> {};
+
+template <Storage ST, Layout LY>
 __device__ __forceinline__ float trilinear(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
                                            float qy, float qz)
 {
@@ -251,32 +313,22 @@ __device__ __forceinline__ float trilinear(__amdgpu_buffer_rsrc_t r, const Pitch
     const float wy = __builtin_amdgcn_fractf(qy);
     const float wz = __builtin_amdgcn_fractf(qz);
     const uint32_t idx = uint32_t(ix) + __umul24(uint32_t(iy), p.ty) + __umul24(uint32_t(iz), p.tz);
-    const uint32_t ofs = idx << Taps<ST>::kShift;
-    const float c00 = Taps<ST>::lerpx(r, ofs, 0, wx);
-    const float c10 = Taps<ST>::lerpx(r, ofs, p.y, wx);
-    const float c01 = Taps<ST>::lerpx(r, ofs, p.z, wx);
-    const float c11 = Taps<ST>::lerpx(r, ofs, p.yz, wx);
-    const float c0 = fmaf(wy, c10 - c00, c00);
-    const float c1 = fmaf(wy, c11 - c01, c01);
-    return fmaf(wz, c1 - c0, c0);
+    return Sampler<ST, LY>::tri(r, p, idx << Sampler<ST, LY>::kShift, wx, wy, wz);
 }
 
-template <Storage ST>
+template <Storage ST, Layout LY>
 __device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
                                          float qy, float qz)
 {
     const uint32_t idx = uint32_t(int(qx)) + __umul24(uint32_t(int(qy)), p.ty) +
                          __umul24(uint32_t(int(qz)), p.tz);
+    constexpr uint32_t shift = LY == Layout::Interleaved ? 3 : (ST == Storage::F32 ? 2 : 1);
     if constexpr (ST == Storage::F32) {
-        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx << 2, 0, 0));
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx << shift, 0, 0));
     } else {
-        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b16(r, idx << 1, 0, 0);
-        if constexpr (ST == Storage::F16) {
-            const _Float16 h = __builtin_bit_cast(_Float16, uint16_t(raw));
-            return float(h);
-        } else {
-            return __uint_as_float(raw << 16);
-        }
+        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b16(r, idx << shift, 0, 0);
+        if constexpr (ST == Storage::F16) return float(__builtin_bit_cast(_Float16, uint16_t(raw)));
+        else return __uint_as_float(raw << 16);
     }
 }
 
@@ -290,7 +342,36 @@ struct CastConst {
     Pitch pitch;
 };
 
-template <Storage ST, bool INTERP, int UNROLL>
+__device__ __forceinline__ int waveMin(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ int waveMax(int v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <Storage ST, Layout LY, bool INTERP>
+__device__ __forceinline__ float sampleAt(__amdgpu_buffer_rsrc_t rs, const Pitch& p, float fi,
+                                          const float s[3], const float c[3])
+{
+    const float qx = fmaf(fi, s[0], c[0]);
+    const float qy = fmaf(fi, s[1], c[1]);
+    const float qz = fmaf(fi, s[2], c[2]);
+    return INTERP ? trilinear<ST, LY>(rs, p, qx, qy, qz) : nearest<ST, LY>(rs, p, qx, qy, qz);
+}
+
+// One thread per detector pixel. All rays of a wavefront step through the same sample indices
+// i (the same distance from the source), so the 64 gathers of one load instruction fall into a
+// small patch of the volume and share cache lines. Each ray's own range [i0, i1] (the reference's
+// loop bounds, trimmed to samples that can touch the volume) is honoured with the exec mask in the
+// head and tail of the wave's range; in between every active ray samples, unmasked and unrolled.
+template <Storage ST, Layout LY, bool INTERP, int UNROLL>
 __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
                                                          RayParams rp, CastConst kc,
                                                          float* __restrict__ out)
@@ -304,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
     const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
     const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
     const uint32_t y = (ty * (kBlock / 64 / kc.bwx) + wy) * kc.wave_h + ly;
-    if (x >= rp.channels || y >= rp.rows) return;
+    const bool pixel = x < rp.channels && y < rp.rows;
 
     const RayGeom g = geom[vm];
     const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
@@ -320,30 +401,39 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
             for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
                 Ray ray = setupRay(g, pcx + float(ra) * ipx, pcy + float(rb) * ipy, kc.inc, kc.vs,
                                    kc.nhi, kc.blo);
+                if (!pixel) ray.hit = false;
                 tighten(ray, g.c, kc.blo, kc.nhi);
-                int n = ray.i1 - ray.i0 + 1;
-                float fi = float(ray.i0);
+                const bool act = pixel && ray.i0 <= ray.i1;
+                const int i0 = act ? ray.i0 : INT_MAX, i1 = act ? ray.i1 : INT_MIN;
+                const int lo = waveMin(i0), hi = waveMax(i1);
+                if (lo > hi) continue; // no ray of this wave touches the volume
+                int a = waveMax(act ? i0 : INT_MIN), bb = waveMin(act ? i1 : INT_MAX);
+                if (a > bb) { a = hi + 1; bb = hi; } // no common range: everything masked
+                const uint32_t span = uint32_t(i1 - i0);
+
                 float part[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
-                for (; n >= UNROLL; n -= UNROLL, fi += float(UNROLL)) {
+                // head [lo, a) and, when there is no common range, everything
+                for (int i = lo; i < a; ++i)
+                    if (act && uint32_t(i - i0) <= span)
+                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, float(i), ray.s, g.c);
+                // body [a, bb]: every active ray samples
+                if (act) {
+                    int i = a;
+                    float fi = float(a);
+                    for (; i + UNROLL - 1 <= bb; i += UNROLL, fi += float(UNROLL)) {
 #pragma unroll
-                    for (int u = 0; u < UNROLL; ++u) {
-                        const float fu = fi + float(u);
-                        const float qx = fmaf(fu, ray.s[0], g.c[0]);
-                        const float qy = fmaf(fu, ray.s[1], g.c[1]);
-                        const float qz = fmaf(fu, ray.s[2], g.c[2]);
-                        part[u] += INTERP ? trilinear<ST>(rs, kc.pitch, qx, qy, qz)
-                                          : nearest<ST>(rs, kc.pitch, qx, qy, qz);
+                        for (int u = 0; u < UNROLL; ++u)
+                            part[u] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, fi + float(u), ray.s, g.c);
                     }
+                    for (; i <= bb; ++i, fi += 1.0f)
+                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, fi, ray.s, g.c);
                 }
-                for (; n > 0; --n, fi += 1.0f) {
-                    const float qx = fmaf(fi, ray.s[0], g.c[0]);
-                    const float qy = fmaf(fi, ray.s[1], g.c[1]);
-                    const float qz = fmaf(fi, ray.s[2], g.c[2]);
-                    part[0] += INTERP ? trilinear<ST>(rs, kc.pitch, qx, qy, qz)
-                                      : nearest<ST>(rs, kc.pitch, qx, qy, qz);
-                }
+                // tail (bb, hi]
+                for (int i = bb + 1; i <= hi; ++i)
+                    if (act && uint32_t(i - i0) <= span)
+                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, float(i), ray.s, g.c);
                 float sum = 0.0f;
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) sum += part[u];
@@ -351,7 +441,8 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
             }
     }
     // raycasterprojector_interp.cl:70-71
-    out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+    if (pixel)
+        out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
 }
 
 __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict__ geom, RayParams rp,
@@ -385,7 +476,8 @@ __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// volume conversion
+// volume conversion: every padded texel (x, y, z) is written from a value function v(x, y, z)
+// of the padded frame that returns 0 outside the interior.
 
 __device__ __forceinline__ uint16_t toHalfBits(float f)
 {
@@ -400,49 +492,90 @@ __device__ __forceinline__ uint16_t toBf16Bits(float f)
     return uint16_t(u >> 16);
 }
 
-__device__ __forceinline__ void storeTexel(void* dst, size_t i, float v, Storage st)
+__device__ __forceinline__ uint16_t toBits(float v, Storage st)
 {
-    if (st == Storage::F32) static_cast<float*>(dst)[i] = v;
-    else if (st == Storage::F16) static_cast<uint16_t*>(dst)[i] = toHalfBits(v);
-    else static_cast<uint16_t*>(dst)[i] = toBf16Bits(v);
+    return st == Storage::F16 ? toHalfBits(v) : toBf16Bits(v);
 }
 
-__device__ __forceinline__ float loadTexel(const void* src, size_t i, Storage st)
+__device__ __forceinline__ float fromBits(uint16_t h, Storage st)
 {
-    if (st == Storage::F32) return static_cast<const float*>(src)[i];
-    const uint16_t h = static_cast<const uint16_t*>(src)[i];
-    if (st == Storage::F16) return float(__builtin_bit_cast(_Float16, h));
-    return __uint_as_float(uint32_t(h) << 16);
+    return st == Storage::F16 ? float(__builtin_bit_cast(_Float16, h)) : __uint_as_float(uint32_t(h) << 16);
 }
 
-__global__ void pack_kernel(const float* __restrict__ src, uint32_t nx, uint32_t ny, uint32_t nz,
-                            uint32_t f, Storage st, void* __restrict__ dst)
+// Value of voxel (x, y, z) of the padded frame stored in a view (first component of its texel).
+__device__ __forceinline__ float viewValue(const VolumeView& s, size_t x, size_t y, size_t z)
 {
-    const size_t px = size_t(nx) * f + 2, py = size_t(ny) * f + 2, pz = size_t(nz) * f + 2;
-    const size_t total = px * py * pz;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
-         i += size_t(gridDim.x) * blockDim.x) {
-        const size_t x = i % px, yz = i / px, y = yz % py, z = yz / py;
-        float v = 0.0f;
-        if (x >= 1 && y >= 1 && z >= 1 && x <= px - 2 && y <= py - 2 && z <= pz - 2)
-            v = src[((z - 1) / f * ny + (y - 1) / f) * nx + (x - 1) / f];
-        storeTexel(dst, i, v, st);
+    const size_t i = z * s.pitch_z + y * s.pitch_y + x;
+    if (s.storage == Storage::F32)
+        return static_cast<const float*>(s.data)[s.layout == Layout::Interleaved ? 2 * i : i];
+    const uint16_t* h = static_cast<const uint16_t*>(s.data);
+    return fromBits(h[s.layout == Layout::Interleaved ? 4 * i : i], s.storage);
+}
+
+template <class F>
+__device__ __forceinline__ void writeTexel(void* dst, size_t i, size_t x, size_t y, size_t z,
+                                           Storage st, Layout ly, const F& v)
+{
+    if (ly == Layout::Plain) {
+        if (st == Storage::F32) static_cast<float*>(dst)[i] = v(x, y, z);
+        else static_cast<uint16_t*>(dst)[i] = toBits(v(x, y, z), st);
+    } else if (st == Storage::F32) {
+        float2 t;
+        t.x = v(x, y, z);
+        t.y = v(x, y, z + 1);
+        static_cast<float2*>(dst)[i] = t;
+    } else {
+        ushort4 t;
+        t.x = toBits(v(x, y, z), st);
+        t.y = toBits(v(x, y + 1, z), st);
+        t.z = toBits(v(x, y, z + 1), st);
+        t.w = toBits(v(x, y + 1, z + 1), st);
+        static_cast<ushort4*>(dst)[i] = t;
     }
 }
 
-__global__ void upsample_kernel(VolumeView s, uint32_t f, void* __restrict__ dst)
+struct FromFloat { // source float array (nx, ny, nz) replicated f times per axis
+    const float* src;
+    uint32_t nx, ny, nz, f;
+    __device__ float operator()(size_t x, size_t y, size_t z) const
+    {
+        if (x < 1 || y < 1 || z < 1 || x > size_t(nx) * f || y > size_t(ny) * f || z > size_t(nz) * f)
+            return 0.0f;
+        return src[((z - 1) / f * ny + (y - 1) / f) * nx + (x - 1) / f];
+    }
+};
+
+struct FromView { // another padded volume replicated f times per axis
+    VolumeView s;
+    uint32_t f;
+    __device__ float operator()(size_t x, size_t y, size_t z) const
+    {
+        if (x < 1 || y < 1 || z < 1 || x > size_t(s.n[0]) * f || y > size_t(s.n[1]) * f ||
+            z > size_t(s.n[2]) * f)
+            return 0.0f;
+        return viewValue(s, (x - 1) / f + 1, (y - 1) / f + 1, (z - 1) / f + 1);
+    }
+};
+
+struct FromMean { // one interior voxel holding sum / count
+    const double* sum;
+    double count;
+    __device__ float operator()(size_t x, size_t y, size_t z) const
+    {
+        // the reference sums in float (raycasterprojector.cpp:603-606); the mean is rounded once
+        return (x == 1 && y == 1 && z == 1) ? float(*sum / count) : 0.0f;
+    }
+};
+
+template <class F>
+__global__ void build_kernel(F v, uint32_t px, uint32_t py, uint32_t pz, Storage st, Layout ly,
+                             void* __restrict__ dst)
 {
-    const size_t px = size_t(s.n[0]) * f + 2, py = size_t(s.n[1]) * f + 2, pz = size_t(s.n[2]) * f + 2;
-    const size_t total = px * py * pz;
+    const size_t total = size_t(px) * py * pz;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += size_t(gridDim.x) * blockDim.x) {
         const size_t x = i % px, yz = i / px, y = yz % py, z = yz / py;
-        float v = 0.0f;
-        if (x >= 1 && y >= 1 && z >= 1 && x <= px - 2 && y <= py - 2 && z <= pz - 2) {
-            const size_t sx = (x - 1) / f + 1, sy = (y - 1) / f + 1, sz = (z - 1) / f + 1;
-            v = loadTexel(s.data, (sz * s.pitch_z) + sy * s.pitch_y + sx, s.storage);
-        }
-        storeTexel(dst, i, v, s.storage);
+        writeTexel(dst, i, x, y, z, st, ly, v);
     }
 }
 
@@ -454,7 +587,7 @@ __global__ void sum_kernel(VolumeView s, double* sum)
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
          i += size_t(gridDim.x) * blockDim.x) {
         const size_t x = i % nx, yz = i / nx, y = yz % ny, z = yz / ny;
-        acc += loadTexel(s.data, (z + 1) * s.pitch_z + (y + 1) * s.pitch_y + x + 1, s.storage);
+        acc += viewValue(s, x + 1, y + 1, z + 1);
     }
     part[threadIdx.x] = acc;
     __syncthreads();
@@ -463,16 +596,6 @@ __global__ void sum_kernel(VolumeView s, double* sum)
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(sum, part[0]);
-}
-
-__global__ void fill_mean_kernel(const double* sum, double count, Storage st, void* dst)
-{
-    const uint32_t i = threadIdx.x;
-    if (i < 27) {
-        // the reference sums in float (raycasterprojector.cpp:603-606); the mean is rounded once
-        const float mean = float(*sum / count);
-        storeTexel(dst, i, i == 13 ? mean : 0.0f, st);
-    }
 }
 
 __global__ void zero_kernel(float* d, size_t n)
@@ -506,7 +629,7 @@ CastConst makeConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams
     }
     kc.blo = rp.interpolate ? 0.0f : 1.0f;
     if (vol) {
-        const uint32_t eb = uint32_t(storageBytes(vol->storage));
+        const uint32_t eb = uint32_t(texelBytes(vol->storage, vol->layout));
         kc.pitch.ty = vol->pitch_y;
         kc.pitch.tz = vol->pitch_z;
         kc.pitch.y = vol->pitch_y * eb;
@@ -516,15 +639,15 @@ CastConst makeConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams
     return kc;
 }
 
-template <Storage ST, bool INTERP>
+template <Storage ST, Layout LY, bool INTERP>
 hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams& rp,
                        const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
                        hipStream_t s)
 {
     switch (unroll) {
-    case 1: raycast_kernel<ST, INTERP, 1><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
-    case 4: raycast_kernel<ST, INTERP, 4><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
-    default: raycast_kernel<ST, INTERP, 2><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    case 4: raycast_kernel<ST, LY, INTERP, 4><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    default: raycast_kernel<ST, LY, INTERP, 2><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
     }
     return hipGetLastError();
 }
@@ -534,8 +657,12 @@ hipError_t launchCastI(const VolumeView& vol, const RayGeom* g, const RayParams&
                        const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
                        hipStream_t s)
 {
-    return rp.interpolate ? launchCastU<ST, true>(vol, g, rp, kc, unroll, blocks, out, s)
-                          : launchCastU<ST, false>(vol, g, rp, kc, unroll, blocks, out, s);
+    if (vol.layout == Layout::Interleaved)
+        return rp.interpolate
+                   ? launchCastU<ST, Layout::Interleaved, true>(vol, g, rp, kc, unroll, blocks, out, s)
+                   : launchCastU<ST, Layout::Interleaved, false>(vol, g, rp, kc, unroll, blocks, out, s);
+    return rp.interpolate ? launchCastU<ST, Layout::Plain, true>(vol, g, rp, kc, unroll, blocks, out, s)
+                          : launchCastU<ST, Layout::Plain, false>(vol, g, rp, kc, unroll, blocks, out, s);
 }
 
 } // namespace
@@ -587,19 +714,21 @@ hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayPar
     return hipGetLastError();
 }
 
-hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, void* d_dst,
-                      hipStream_t s)
+hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Storage st, Layout ly,
+                      void* d_dst, hipStream_t s)
 {
-    const size_t total = (size_t(n[0]) * f + 2) * (size_t(n[1]) * f + 2) * (size_t(n[2]) * f + 2);
-    pack_kernel<<<gridFor(total), kBlock, 0, s>>>(d_src, n[0], n[1], n[2], f, st, d_dst);
+    const uint32_t px = n[0] * f + 2, py = n[1] * f + 2, pz = n[2] * f + 2;
+    const FromFloat v{ d_src, n[0], n[1], n[2], f };
+    build_kernel<<<gridFor(size_t(px) * py * pz), kBlock, 0, s>>>(v, px, py, pz, st, ly, d_dst);
     return hipGetLastError();
 }
 
 hipError_t launchUpsample(const VolumeView& src, uint32_t f, void* d_dst, hipStream_t s)
 {
-    const size_t total =
-        (size_t(src.n[0]) * f + 2) * (size_t(src.n[1]) * f + 2) * (size_t(src.n[2]) * f + 2);
-    upsample_kernel<<<gridFor(total), kBlock, 0, s>>>(src, f, d_dst);
+    const uint32_t px = src.n[0] * f + 2, py = src.n[1] * f + 2, pz = src.n[2] * f + 2;
+    const FromView v{ src, f };
+    build_kernel<<<gridFor(size_t(px) * py * pz), kBlock, 0, s>>>(v, px, py, pz, src.storage,
+                                                                  src.layout, d_dst);
     return hipGetLastError();
 }
 
@@ -610,9 +739,11 @@ hipError_t launchSum(const VolumeView& src, double* d_sum, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launchFillMean(const double* d_sum, double count, Storage st, void* d_dst, hipStream_t s)
+hipError_t launchFillMean(const double* d_sum, double count, Storage st, Layout ly, void* d_dst,
+                          hipStream_t s)
 {
-    fill_mean_kernel<<<1, 64, 0, s>>>(d_sum, count, st, d_dst);
+    const FromMean v{ d_sum, count };
+    build_kernel<<<1, 64, 0, s>>>(v, 3u, 3u, 3u, st, ly, d_dst);
     return hipGetLastError();
 }
 

@@ -1,0 +1,200 @@
+// ctlproj.hpp -- header-only C++ mirror of CTL's projector interface over the C ABI (ctlproj.h).
+//
+// Mirrors CTL::OCL::RayCasterProjector (modules/src/projectors/raycasterprojector.h:20-58) and
+// AbstractProjector (modules/src/projectors/abstractprojector.h:65-112) without Qt:
+//   RayCasterProjector p;  p.settings().raySampling = 0.2f;
+//   p.configure(pmats, viewDim);            // FullGeometry as [views][modules][3][4] doubles
+//   ProjectionData d = p.project(volume);   // VoxelVolume<float>, x fastest
+// Errors follow the reference: an invalid device throws std::runtime_error (as the reference's
+// constructor and prepareOpenCLDeviceList do); a volume without data returns ProjectionData with 0
+// views; other failures throw std::runtime_error carrying ctlproj_last_error().
+#ifndef CTLPROJ_HPP
+#define CTLPROJ_HPP
+
+#include "ctlproj.h"
+
+#include <array>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ctl_amd {
+
+inline void check(ctlproj_status st)
+{
+    if (st != CTLPROJ_OK)
+        throw std::runtime_error(std::string("ctlproj error ") + std::to_string(int(st)) + ": " +
+                                 ctlproj_last_error());
+}
+
+// VoxelVolume<float> (img/voxelvolume.h): dims X, Y, Z; data x fastest; voxel size and offset in mm.
+struct VoxelVolume {
+    std::array<uint32_t, 3> dims{ { 0, 0, 0 } };
+    std::array<float, 3> voxelSize{ { 1.0f, 1.0f, 1.0f } };
+    std::array<float, 3> offset{ { 0.0f, 0.0f, 0.0f } };
+    std::vector<float> data;
+    bool hasData() const
+    {
+        return !data.empty() && data.size() == size_t(dims[0]) * dims[1] * dims[2];
+    }
+};
+
+// SingleViewData::Dimensions (img/singleviewdata.h).
+struct ViewDimensions {
+    uint32_t nbChannels = 0, nbRows = 0, nbModules = 0;
+};
+
+// ProjectionData (img/projectiondata.h): views -> modules -> rows -> channels, channel fastest.
+struct ProjectionData {
+    ViewDimensions viewDim;
+    uint32_t nbViews = 0;
+    std::vector<float> data;
+    float& operator()(uint32_t view, uint32_t module, uint32_t x, uint32_t y)
+    {
+        return data[((size_t(view) * viewDim.nbModules + module) * viewDim.nbRows + y) *
+                        viewDim.nbChannels + x];
+    }
+};
+
+// One context per device (the replacement of OpenCLConfig::instance()).
+class Context {
+public:
+    explicit Context(int device = 0) { check(ctlproj_context_create(device, &h_)); }
+    ~Context() { ctlproj_context_destroy(h_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    ctlproj_context* get() const { return h_; }
+
+private:
+    ctlproj_context* h_ = nullptr;
+};
+
+// A volume kept resident in HBM across projections.
+class ResidentVolume {
+public:
+    ResidentVolume(Context& ctx, const VoxelVolume& v, ctlproj_dtype storage = CTLPROJ_F32)
+    {
+        check(ctlproj_volume_create(ctx.get(), v.data.data(), v.dims.data(), v.voxelSize.data(),
+                                    v.offset.data(), storage, &h_));
+    }
+    ~ResidentVolume() { ctlproj_volume_destroy(h_); }
+    ResidentVolume(const ResidentVolume&) = delete;
+    ResidentVolume& operator=(const ResidentVolume&) = delete;
+    const ctlproj_volume* get() const { return h_; }
+
+private:
+    ctlproj_volume* h_ = nullptr;
+};
+
+class RayCasterProjector {
+public:
+    // RayCasterProjector::Settings (raycasterprojector.h:27-37)
+    struct Settings {
+        uint32_t raysPerPixel[2] = { 1, 1 };
+        float raySampling = 0.3f;
+        uint32_t volumeUpSampling = 1;
+        bool interpolate = true;
+    };
+
+    explicit RayCasterProjector(std::shared_ptr<Context> ctx = std::make_shared<Context>(0))
+        : ctx_(std::move(ctx))
+    {
+        ctlproj_settings s;
+        ctlproj_settings_default(&s);
+        check(ctlproj_projector_create(ctx_->get(), &s, &h_));
+    }
+    ~RayCasterProjector() { ctlproj_projector_destroy(h_); }
+    RayCasterProjector(const RayCasterProjector&) = delete;
+    RayCasterProjector& operator=(const RayCasterProjector&) = delete;
+
+    Settings& settings() { return settings_; }
+    bool isLinear() const { return true; }
+
+    // ProjectorNotifier::projectionFinished(int) (abstractprojector.h:57-63)
+    std::function<void(int)> projectionFinished;
+
+    // configure (raycasterprojector.cpp:97-107) from an encoded FullGeometry
+    void configure(const std::vector<double>& pmats, const ViewDimensions& viewDim)
+    {
+        const size_t per_view = size_t(viewDim.nbModules) * 12;
+        if (per_view == 0 || pmats.size() % per_view != 0)
+            throw std::domain_error("FullGeometry size does not match the view dimensions");
+        viewDim_ = viewDim;
+        nbViews_ = uint32_t(pmats.size() / per_view);
+        check(ctlproj_projector_configure(h_, pmats.data(), nbViews_, viewDim.nbModules,
+                                          viewDim.nbChannels, viewDim.nbRows));
+    }
+
+    // project (raycasterprojector.cpp:139-375): uploads the volume, as the reference does
+    ProjectionData project(const VoxelVolume& volume)
+    {
+        ProjectionData ret;
+        ret.viewDim = viewDim_;
+        if (!volume.hasData()) return ret; // the reference returns 0 views (.cpp:156-160)
+        pushSettings();
+        ret.nbViews = nbViews_;
+        ret.data.resize(size_t(nbViews_) * viewDim_.nbModules * viewDim_.nbRows * viewDim_.nbChannels);
+        check(ctlproj_project_host_volume(h_, volume.data.data(), volume.dims.data(),
+                                          volume.voxelSize.data(), volume.offset.data(),
+                                          ret.data.data(), ret.data.size()));
+        if (projectionFinished)
+            for (uint32_t v = 0; v < nbViews_; ++v) projectionFinished(int(v));
+        return ret;
+    }
+
+    // project from a volume that stays resident on the device
+    ProjectionData project(const ResidentVolume& volume)
+    {
+        pushSettings();
+        ProjectionData ret;
+        ret.viewDim = viewDim_;
+        ret.nbViews = nbViews_;
+        ret.data.resize(size_t(nbViews_) * viewDim_.nbModules * viewDim_.nbRows * viewDim_.nbChannels);
+        check(ctlproj_project(h_, volume.get(), ret.data.data(), ret.data.size(), &progressThunk, this));
+        return ret;
+    }
+
+    // projectComposite (abstractprojector.h:99-112): the sum of the sub-volume projections
+    ProjectionData projectComposite(const std::vector<VoxelVolume>& volumes)
+    {
+        ProjectionData sum;
+        for (const auto& v : volumes) {
+            ProjectionData p = project(v);
+            if (sum.data.empty()) sum = std::move(p);
+            else
+                for (size_t i = 0; i < sum.data.size() && i < p.data.size(); ++i) sum.data[i] += p.data[i];
+        }
+        return sum;
+    }
+
+private:
+    static void progressThunk(int view, void* self)
+    {
+        auto* p = static_cast<RayCasterProjector*>(self);
+        if (p->projectionFinished) p->projectionFinished(view);
+    }
+
+    void pushSettings()
+    {
+        ctlproj_settings s;
+        ctlproj_settings_default(&s);
+        s.rays_per_pixel[0] = settings_.raysPerPixel[0];
+        s.rays_per_pixel[1] = settings_.raysPerPixel[1];
+        s.ray_sampling = settings_.raySampling;
+        s.volume_upsampling = settings_.volumeUpSampling;
+        s.interpolate = settings_.interpolate ? 1 : 0;
+        check(ctlproj_projector_set_settings(h_, &s));
+    }
+
+    std::shared_ptr<Context> ctx_;
+    ctlproj_projector* h_ = nullptr;
+    Settings settings_;
+    ViewDimensions viewDim_;
+    uint32_t nbViews_ = 0;
+};
+
+} // namespace ctl_amd
+
+#endif // CTLPROJ_HPP

@@ -210,3 +210,20 @@ def test_progress_callback_order():
     p.progress = seen.append
     p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm)))
     assert seen == list(range(36))
+
+
+def test_cpp_mirror_program(tmp_path):
+    """The header-only C++ mirror (include/ctlproj.hpp), driven like CTL code, matches ref32."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "ctl_amd" / "lib" / "projector_demo"
+    out = tmp_path / "proj.f32"
+    res = subprocess.run([str(exe), str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "views=36 progress=36 empty_ok=1" in res.stdout
+    got = np.fromfile(out, np.float32).reshape(36, 1, 128, 256)
+    w = W.small()
+    r32, _ = run_ref(w)
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"C++ mirror: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= L2_F32 and m <= MAR_F32

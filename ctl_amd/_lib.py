@@ -34,6 +34,7 @@ class Settings(C.Structure):
         ("tile_w", C.c_uint32),
         ("tile_h", C.c_uint32),
         ("unroll", C.c_uint32),
+        ("blocks_per_cu", C.c_uint32),
     ]
 
 
@@ -105,6 +106,18 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_phantom_ellipsoids": (_i, [_u32p, _f32p, _u32, C.c_uint64, C.c_float, C.c_float,
                                         C.c_float, C.c_float, C.c_float, _f32p]),
     "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
+    "ctlproj_projector_autotune": (_i, [_vp, _vp, _f32p]),
+    "ctlproj_autotune_table": (_i, [C.c_char_p, _sz, C.POINTER(_sz)]),
+    "ctlproj_batch_create": (_i, [_vp, _pp]),
+    "ctlproj_batch_destroy": (_i, [_vp]),
+    "ctlproj_batch_add": (_i, [_vp, _vp, _f64p, _u32, _u32, _u32, _u32, C.POINTER(Settings), _vp, _sz,
+                               C.POINTER(_u32)]),
+    "ctlproj_batch_finalize": (_i, [_vp, _i]),
+    "ctlproj_batch_launch": (_i, [_vp]),
+    "ctlproj_batch_synchronize": (_i, [_vp]),
+    "ctlproj_batch_last_ms": (_i, [_vp, _f32p]),
+    "ctlproj_batch_info": (_i, [_vp, _u32p, _u32p, _u32p, _u32p, _u64p]),
+    "ctlproj_batch_count_samples": (_i, [_vp, _u64p]),
 }
 
 _lib: C.CDLL | None = None

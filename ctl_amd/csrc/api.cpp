@@ -3,6 +3,7 @@
 // Replaces the OpenCL environment of CTL (OpenCLConfig, openclconfig.cpp:15-171; ClFileLoader;
 // PinnedBuf*) and RayCasterProjector::project (raycasterprojector.cpp:139-375). Code objects are
 // compiled ahead of time for gfx950, so there is no runtime kernel build.
+#include "autotune.h"
 #include "ctlproj.h"
 #include "geometry.h"
 #include "kernels.h"
@@ -71,17 +72,23 @@ ctlproj_status validateSettings(const ctlproj_settings& s)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "tile_w * tile_h must be 64 (one wavefront)");
     if (s.unroll != 0 && s.unroll != 1 && s.unroll != 2 && s.unroll != 4)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unroll must be 0, 1, 2 or 4");
+    if (s.blocks_per_cu > 8)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "blocks_per_cu must be 0..8");
     return CTLPROJ_OK;
 }
 
-ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage, uint32_t, uint32_t)
+ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly,
+                           uint32_t channels, uint32_t rows)
 {
+    // explicit settings win; otherwise the autotuner's choice for this workload class, if any
     ctlk::LaunchShape sh;
+    ctltune::lookup(ctltune::key(st, ly, s.interpolate ? 1 : 0, channels, rows), &sh);
     if (s.tile_w && s.tile_h) {
         sh.wave_w = s.tile_w;
         sh.wave_h = s.tile_h;
     }
     if (s.unroll) sh.unroll = s.unroll;
+    if (s.blocks_per_cu) sh.blocks_per_cu = s.blocks_per_cu;
     return sh;
 }
 
@@ -289,7 +296,7 @@ ctlproj_status launchCast(ctlproj_projector* p, const EffectiveVolume& ev, uint3
                           float* d_out)
 {
     const ctlk::RayParams rp = rayParams(p, nv);
-    const ctlk::LaunchShape sh = shapeFor(p->s, ev.view.storage, p->C, p->R);
+    const ctlk::LaunchShape sh = shapeFor(p->s, ev.view.storage, ev.view.layout, p->C, p->R);
     hipError_t e = ctlk::launchRaycast(ev.view, ev.geom, p->d_geom + size_t(v0) * p->M, rp, sh,
                                        d_out, p->stream);
     return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "raycast kernel");
@@ -666,6 +673,54 @@ ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlpr
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) return hipFail(e, "count kernel");
     *n_samples = h;
+    return CTLPROJ_OK;
+}
+
+ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_volume* vol,
+                                         float* best_ms)
+{
+    ctlproj_status st = checkProject(p, vol, nullptr, p ? size_t(p->V) * p->M * p->R * p->C : 0, false);
+    if (st != CTLPROJ_OK) return st;
+    if (p->V == 0) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "no views to tune on");
+    if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+    EffectiveVolume ev;
+    if ((st = effectiveVolume(vol, p->s, p->stream, ev)) != CTLPROJ_OK) return st;
+    if ((st = prepareGeometry(p, ev)) != CTLPROJ_OK) return st;
+    // time on up to 24 evenly spaced views, gathered into a contiguous geometry block
+    const uint32_t nv = std::min<uint32_t>(p->V, 24);
+    const size_t per_view = size_t(p->M) * p->R * p->C;
+    ctlk::RayGeom* d_g = nullptr;
+    float* d_o = nullptr;
+    hipError_t e = hipMalloc(&d_g, size_t(nv) * p->M * sizeof(ctlk::RayGeom));
+    if (e == hipSuccess) e = hipMalloc(&d_o, size_t(nv) * per_view * sizeof(float));
+    for (uint32_t k = 0; k < nv && e == hipSuccess; ++k) {
+        const uint32_t v = uint32_t(uint64_t(k) * p->V / nv);
+        e = hipMemcpyAsync(d_g + size_t(k) * p->M, p->d_geom + size_t(v) * p->M, p->M * sizeof(ctlk::RayGeom),
+                           hipMemcpyDeviceToDevice, p->stream);
+    }
+    if (e != hipSuccess) {
+        if (d_g) (void)hipFree(d_g);
+        if (d_o) (void)hipFree(d_o);
+        return hipFail(e, "autotune buffers");
+    }
+    const ctlk::RayParams rp = rayParams(p, nv);
+    auto run = [&](const ctlk::LaunchShape& sh) -> float {
+        hipError_t r = hipEventRecord(p->ev_start, p->stream);
+        if (r == hipSuccess) r = ctlk::launchRaycast(ev.view, ev.geom, d_g, rp, sh, d_o, p->stream);
+        if (r == hipSuccess) r = hipEventRecord(p->ev_stop, p->stream);
+        if (r == hipSuccess) r = hipEventSynchronize(p->ev_stop);
+        float ms = -1.0f;
+        if (r == hipSuccess && hipEventElapsedTime(&ms, p->ev_start, p->ev_stop) != hipSuccess) ms = -1.0f;
+        return r == hipSuccess ? ms : -1.0f;
+    };
+    ctltune::Choice best;
+    const bool ok = ctltune::tune(ctltune::key(ev.view.storage, ev.view.layout, p->s.interpolate ? 1 : 0,
+                                               p->C, p->R),
+                                  run, &best);
+    (void)hipFree(d_g);
+    (void)hipFree(d_o);
+    if (!ok) return fail(CTLPROJ_ERR_DEVICE, "autotuning failed: no candidate ran");
+    if (best_ms) *best_ms = best.ms;
     return CTLPROJ_OK;
 }
 

@@ -1,0 +1,92 @@
+// Autotuner for the ray-casting launch shape: measures candidate (pixel tile, unroll) shapes on
+// the caller's own workload (or a sample of it) and remembers the fastest per workload class.
+#include "autotune.h"
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+namespace ctltune {
+
+namespace {
+std::mutex g_mu;
+std::map<std::string, Choice> g_table;
+} // namespace
+
+std::vector<ctlk::LaunchShape> candidates()
+{
+    std::vector<ctlk::LaunchShape> out;
+    const uint32_t tiles[3][2] = { { 8, 8 }, { 16, 4 }, { 4, 16 } };
+    for (const auto& t : tiles)
+        for (uint32_t u : { 2u, 4u })
+            for (uint32_t occ : { 0u, 4u }) {
+                ctlk::LaunchShape s;
+                s.wave_w = t[0];
+                s.wave_h = t[1];
+                s.block_waves_x = 2;
+                s.block_waves_y = 2;
+                s.unroll = u;
+                s.blocks_per_cu = occ;
+                out.push_back(s);
+            }
+    return out;
+}
+
+bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
+          Choice* best)
+{
+    Choice b;
+    b.ms = -1.0f;
+    for (const auto& s : candidates()) {
+        run(s); // first launch of a shape warms its code object and caches
+        const float ms = run(s);
+        if (ms < 0.0f) continue;
+        if (b.ms < 0.0f || ms < b.ms) {
+            b.shape = s;
+            b.ms = ms;
+        }
+    }
+    if (b.ms < 0.0f) return false;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[key] = b;
+    if (best) *best = b;
+    return true;
+}
+
+bool lookup(const std::string& key, ctlk::LaunchShape* shape)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_table.find(key);
+    if (it == g_table.end()) return false;
+    *shape = it->second.shape;
+    return true;
+}
+
+std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows)
+{
+    // detector size class: number of 256-pixel steps of the larger side
+    const uint32_t cls = ((channels > rows ? channels : rows) + 255) / 256;
+    std::ostringstream os;
+    os << "s" << int(st) << "_l" << int(ly) << "_i" << interpolate << "_d" << cls;
+    return os.str();
+}
+
+std::string exportTable()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::ostringstream os;
+    os << "{";
+    bool first = true;
+    for (const auto& kv : g_table) {
+        if (!first) os << ",";
+        first = false;
+        const auto& s = kv.second.shape;
+        os << "\"" << kv.first << "\":{\"tile_w\":" << s.wave_w << ",\"tile_h\":" << s.wave_h
+           << ",\"unroll\":" << s.unroll << ",\"blocks_per_cu\":" << s.blocks_per_cu << ",\"ms\":" << kv.second.ms << "}";
+    }
+    os << "}";
+    return os.str();
+}
+
+} // namespace ctltune

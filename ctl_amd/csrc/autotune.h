@@ -1,0 +1,35 @@
+// Autotuner for the ray-casting launch shape (wavefront pixel tile and ray-step unroll).
+#pragma once
+
+#include "kernels.h"
+
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace ctltune {
+
+struct Choice {
+    ctlk::LaunchShape shape;
+    float ms = 0.0f;
+};
+
+// Candidate shapes: wavefront tiles 8x8, 16x4, 4x16 (pixels along channels x rows), unroll 2, 4
+// and a cap of 0 (none) or 4 resident workgroups per CU.
+std::vector<ctlk::LaunchShape> candidates();
+
+// Times every candidate with `run` (returns milliseconds, or a negative value on failure), keeps
+// the fastest and records it in the process-wide table under `key`.
+bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
+          Choice* best);
+
+// Shape recorded for `key` by an earlier tune(); false if none.
+bool lookup(const std::string& key, ctlk::LaunchShape* shape);
+
+// Key of a workload class: storage, layout, interpolation and detector size class.
+std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows);
+
+// The whole table as JSON text.
+std::string exportTable();
+
+} // namespace ctltune

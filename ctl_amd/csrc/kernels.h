@@ -60,13 +60,65 @@ struct LaunchShape {
     uint32_t wave_w = 8, wave_h = 8; // pixel tile per wavefront (wave_w * wave_h = 64)
     uint32_t block_waves_x = 2, block_waves_y = 2; // waves per workgroup
     uint32_t unroll = 4;
+    uint32_t blocks_per_cu = 0; // 0: as many as registers allow; else capped with reserved LDS
 };
+
+// Dynamic LDS that caps the resident workgroups per CU at `blocks` (160 KiB of LDS per CU).
+inline uint32_t ldsForBlocks(uint32_t blocks)
+{
+    return blocks ? ((160u * 1024u / blocks) / 256u) * 256u : 0u;
+}
 
 struct RayParams {
     uint32_t channels, rows, modules, views;
     uint32_t rpp_x, rpp_y;
     int interpolate;
 };
+
+struct Pitch {
+    uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) texel rows
+    uint32_t ty, tz;   // texels per row / slice
+};
+
+// Launch constants of one projection (tile decomposition, step, box, texel pitches).
+struct CastConst {
+    uint32_t tiles_x, tiles_y, tiles_per_vm;
+    uint32_t wave_w, wave_h, bwx, bwy;
+    float inc;
+    float vs[3];
+    float nhi[3]; // n + 1 per axis
+    float blo;    // 0 (trilinear) or 1 (nearest)
+    Pitch pitch;
+};
+
+CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
+                        const LaunchShape& shape);
+
+// One projection of a batch: volume, per-view geometry, output and its first tile in the batch.
+struct BatchItem {
+    VolumeView vol;
+    const RayGeom* geom;
+    float* out;
+    RayParams rp;
+    CastConst kc;
+    unsigned long long tile_begin;
+};
+
+// Work queue of the persistent batch kernel: one tile range per XCD.
+struct BatchQueue {
+    unsigned long long next[8];
+    unsigned long long end[8];
+};
+
+// Resets the queue to [0, total) split over the XCDs (a kernel, so it can sit in a graph).
+hipError_t launchQueueInit(BatchQueue* d_q, unsigned long long total, hipStream_t s);
+// Persistent kernel over `n_items` projections; all items share storage, layout and mode.
+hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, Layout ly,
+                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t grid,
+                       BatchQueue* d_q, uint32_t stride, hipStream_t s);
+// Workgroups of the batch kernel that fit on the device at once.
+hipError_t batchGrid(Storage st, Layout ly, int interpolate, uint32_t unroll, uint32_t blocks_per_cu,
+                     uint32_t* grid);
 
 // FP64 setup: one thread per (view, module).
 hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,

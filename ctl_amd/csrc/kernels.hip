@@ -204,10 +204,7 @@ __device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, con
 // ---------------------------------------------------------------------------------------------
 // trilinear / nearest sample at texel-frame position q (padded frame, blo <= q < n+1)
 
-struct Pitch {
-    uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) texel rows
-    uint32_t ty, tz;   // texels per row / slice
-};
+
 
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
 
@@ -332,15 +329,7 @@ __device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& 
     }
 }
 
-struct CastConst {
-    uint32_t tiles_x, tiles_y, tiles_per_vm;
-    uint32_t wave_w, wave_h, bwx;
-    float inc;
-    float vs[3];
-    float nhi[3]; // n + 1 per axis
-    float blo;    // 0 (trilinear) or 1 (nearest)
-    Pitch pitch;
-};
+
 
 __device__ __forceinline__ int waveMin(int v)
 {
@@ -371,20 +360,25 @@ __device__ __forceinline__ float sampleAt(__amdgpu_buffer_rsrc_t rs, const Pitch
 // small patch of the volume and share cache lines. Each ray's own range [i0, i1] (the reference's
 // loop bounds, trimmed to samples that can touch the volume) is honoured with the exec mask in the
 // head and tail of the wave's range; in between every active ray samples, unmasked and unrolled.
+// Processes logical tile b of one projection: one thread per detector pixel. All rays of a
+// wavefront step through the same sample indices i (the same distance from the source), so the
+// 64 gathers of one load instruction fall into a small patch of the volume and share cache lines.
+// Each ray's own range [i0, i1] (the reference's loop bounds, trimmed to samples that can touch
+// the volume) is honoured with the exec mask in the head and tail of the wave's range; in between
+// every active ray samples, unmasked and unrolled.
 template <Storage ST, Layout LY, bool INTERP, int UNROLL>
-__global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
-                                                         RayParams rp, CastConst kc,
-                                                         float* __restrict__ out)
+__device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* __restrict__ geom,
+                                         const RayParams& rp, const CastConst& kc,
+                                         float* __restrict__ out, uint32_t b, uint32_t wave)
 {
-    const uint32_t b = xcdRemap(blockIdx.x, gridDim.x);
     const uint32_t vm = b / kc.tiles_per_vm;
     const uint32_t t = b - vm * kc.tiles_per_vm;
     const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
     const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
     const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
-    const uint32_t y = (ty * (kBlock / 64 / kc.bwx) + wy) * kc.wave_h + ly;
+    const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
 
     const RayGeom g = geom[vm];
@@ -443,6 +437,76 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
     // raycasterprojector_interp.cl:70-71
     if (pixel)
         out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+}
+
+template <Storage ST, Layout LY, bool INTERP, int UNROLL>
+__global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
+                                                         RayParams rp, CastConst kc,
+                                                         float* __restrict__ out)
+{
+    castTile<ST, LY, INTERP, UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x),
+                                     threadIdx.x >> 6);
+}
+
+// Persistent batch kernel: a fixed grid of workgroups drains a queue of tiles that spans many
+// projections (volumes, geometries and detector sizes of their own). A tile is a workgroup's
+// 2x2 wavefronts of neighbouring pixels, so the four waves on a CU gather from the same patch
+// of the volume. The tile space is cut into one contiguous range per XCD; a workgroup takes tiles
+// from its own XCD's range first (neighbouring tiles of one view then share that XCD's L2) and
+// then from the others. `stride` > 1 visits every stride-th tile (used by the autotuner to time a
+// sample of the work).
+template <Storage ST, Layout LY, bool INTERP, int UNROLL>
+__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem* __restrict__ items, uint32_t n_items,
+                                                       BatchQueue* __restrict__ q, uint32_t stride)
+{
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_item;
+    const uint32_t home = blockIdx.x % kNumXcd;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            unsigned long long got = ~0ull;
+            for (uint32_t k = 0; k < kNumXcd && got == ~0ull; ++k) {
+                const uint32_t x = (home + k) % kNumXcd;
+                if (__hip_atomic_load(&q->next[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q->end[x]) continue;
+                const unsigned long long t = atomicAdd(&q->next[x], 1ull);
+                if (t < q->end[x]) got = t;
+            }
+            uint32_t lo = 0;
+            if (got != ~0ull) {
+                const unsigned long long tt = got * stride;
+                uint32_t hi = n_items; // last item with tile_begin <= tt
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (items[mid].tile_begin <= tt) lo = mid;
+                    else hi = mid;
+                }
+            }
+            s_tile = got;
+            s_item = lo;
+        }
+        __syncthreads();
+        // wave-uniform copies: the item's fields are then scalar loads
+        const uint32_t tlo = __builtin_amdgcn_readfirstlane(uint32_t(s_tile));
+        const uint32_t thi = __builtin_amdgcn_readfirstlane(uint32_t(s_tile >> 32));
+        const uint32_t k = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        const unsigned long long t = (unsigned long long)thi << 32 | tlo;
+        if (t == ~0ull) break; // every workgroup exits once all ranges are drained
+        const BatchItem& it = items[k];
+        castTile<ST, LY, INTERP, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out,
+                                         uint32_t(t * stride - it.tile_begin), threadIdx.x >> 6);
+    }
+}
+
+__global__ void queue_init_kernel(BatchQueue* q, unsigned long long total)
+{
+    const uint32_t x = threadIdx.x;
+    if (x < kNumXcd) {
+        const unsigned long long per = total / kNumXcd, rem = total % kNumXcd;
+        const unsigned long long begin = x * per + (x < rem ? x : rem);
+        q->next[x] = begin;
+        q->end[x] = begin + per + (x < rem ? 1 : 0);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict__ geom, RayParams rp,
@@ -611,13 +675,16 @@ uint32_t gridFor(size_t n)
     return uint32_t(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
 }
 
-CastConst makeConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
+} // namespace
+
+CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
                     const LaunchShape& sh)
 {
     CastConst kc{};
     kc.wave_w = sh.wave_w;
     kc.wave_h = sh.wave_h;
     kc.bwx = sh.block_waves_x;
+    kc.bwy = sh.block_waves_y;
     const uint32_t btw = sh.wave_w * sh.block_waves_x, bth = sh.wave_h * sh.block_waves_y;
     kc.tiles_x = (rp.channels + btw - 1) / btw;
     kc.tiles_y = (rp.rows + bth - 1) / bth;
@@ -639,30 +706,32 @@ CastConst makeConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams
     return kc;
 }
 
+namespace {
+
 template <Storage ST, Layout LY, bool INTERP>
 hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams& rp,
-                       const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
+                       const CastConst& kc, uint32_t unroll, uint32_t blocks, uint32_t lds, float* out,
                        hipStream_t s)
 {
     switch (unroll) {
-    case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
-    case 4: raycast_kernel<ST, LY, INTERP, 4><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
-    default: raycast_kernel<ST, LY, INTERP, 2><<<blocks, kBlock, 0, s>>>(vol, g, rp, kc, out); break;
+    case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    case 4: raycast_kernel<ST, LY, INTERP, 4><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    default: raycast_kernel<ST, LY, INTERP, 2><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     }
     return hipGetLastError();
 }
 
 template <Storage ST>
 hipError_t launchCastI(const VolumeView& vol, const RayGeom* g, const RayParams& rp,
-                       const CastConst& kc, uint32_t unroll, uint32_t blocks, float* out,
+                       const CastConst& kc, uint32_t unroll, uint32_t blocks, uint32_t lds, float* out,
                        hipStream_t s)
 {
     if (vol.layout == Layout::Interleaved)
         return rp.interpolate
-                   ? launchCastU<ST, Layout::Interleaved, true>(vol, g, rp, kc, unroll, blocks, out, s)
-                   : launchCastU<ST, Layout::Interleaved, false>(vol, g, rp, kc, unroll, blocks, out, s);
-    return rp.interpolate ? launchCastU<ST, Layout::Plain, true>(vol, g, rp, kc, unroll, blocks, out, s)
-                          : launchCastU<ST, Layout::Plain, false>(vol, g, rp, kc, unroll, blocks, out, s);
+                   ? launchCastU<ST, Layout::Interleaved, true>(vol, g, rp, kc, unroll, blocks, lds, out, s)
+                   : launchCastU<ST, Layout::Interleaved, false>(vol, g, rp, kc, unroll, blocks, lds, out, s);
+    return rp.interpolate ? launchCastU<ST, Layout::Plain, true>(vol, g, rp, kc, unroll, blocks, lds, out, s)
+                          : launchCastU<ST, Layout::Plain, false>(vol, g, rp, kc, unroll, blocks, lds, out, s);
 }
 
 } // namespace
@@ -670,7 +739,7 @@ hipError_t launchCastI(const VolumeView& vol, const RayGeom* g, const RayParams&
 bool validShape(const LaunchShape& s)
 {
     return s.wave_w * s.wave_h == 64 && s.block_waves_x * s.block_waves_y == kBlock / 64 &&
-           (s.unroll == 1 || s.unroll == 2 || s.unroll == 4);
+           (s.unroll == 1 || s.unroll == 2 || s.unroll == 4) && s.blocks_per_cu <= 8;
 }
 
 hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,
@@ -691,21 +760,22 @@ hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayG
                          hipStream_t s)
 {
     if (!validShape(shape)) return hipErrorInvalidValue;
-    const CastConst kc = makeConst(&vol, vg, rp, shape);
+    const CastConst kc = makeCastConst(&vol, vg, rp, shape);
     const uint64_t blocks = uint64_t(kc.tiles_per_vm) * rp.views * rp.modules;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
+    const uint32_t lds = ldsForBlocks(shape.blocks_per_cu);
     switch (vol.storage) {
-    case Storage::F16: return launchCastI<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
-    case Storage::BF16: return launchCastI<Storage::BF16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
-    default: return launchCastI<Storage::F32>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), d_out, s);
+    case Storage::F16: return launchCastI<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
+    case Storage::BF16: return launchCastI<Storage::BF16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
+    default: return launchCastI<Storage::F32>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
     }
 }
 
 hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
                        unsigned long long* d_count, hipStream_t s)
 {
-    const CastConst kc = makeConst(nullptr, vg, rp, LaunchShape{});
+    const CastConst kc = makeCastConst(nullptr, vg, rp, LaunchShape{});
     const uint64_t rays = uint64_t(rp.channels) * rp.rows * rp.modules * rp.views;
     const uint64_t blocks = (rays + kBlock - 1) / kBlock;
     if (blocks == 0) return hipSuccess;
@@ -751,6 +821,66 @@ hipError_t launchZero(float* d, size_t n, hipStream_t s)
 {
     zero_kernel<<<gridFor(n), kBlock, 0, s>>>(d, n);
     return hipGetLastError();
+}
+
+hipError_t launchQueueInit(BatchQueue* d_q, unsigned long long total, hipStream_t s)
+{
+    queue_init_kernel<<<1, 64, 0, s>>>(d_q, total);
+    return hipGetLastError();
+}
+
+namespace {
+template <Storage ST, Layout LY, bool INTERP>
+const void* batchKernelU(uint32_t unroll)
+{
+    switch (unroll) {
+    case 1: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 1>);
+    case 4: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 4>);
+    default: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 2>);
+    }
+}
+
+template <Storage ST>
+const void* batchKernelS(Layout ly, int interp, uint32_t unroll)
+{
+    if (ly == Layout::Interleaved)
+        return interp ? batchKernelU<ST, Layout::Interleaved, true>(unroll)
+                      : batchKernelU<ST, Layout::Interleaved, false>(unroll);
+    return interp ? batchKernelU<ST, Layout::Plain, true>(unroll) : batchKernelU<ST, Layout::Plain, false>(unroll);
+}
+
+const void* batchKernel(Storage st, Layout ly, int interp, uint32_t unroll)
+{
+    switch (st) {
+    case Storage::F16: return batchKernelS<Storage::F16>(ly, interp, unroll);
+    case Storage::BF16: return batchKernelS<Storage::BF16>(ly, interp, unroll);
+    default: return batchKernelS<Storage::F32>(ly, interp, unroll);
+    }
+}
+} // namespace
+
+hipError_t batchGrid(Storage st, Layout ly, int interpolate, uint32_t unroll, uint32_t blocks_per_cu,
+                     uint32_t* grid)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batchKernel(st, ly, interpolate, unroll),
+                                                         kBlock, ldsForBlocks(blocks_per_cu));
+    if (e != hipSuccess) return e;
+    *grid = uint32_t(cus * (per_cu > 0 ? per_cu : 1));
+    return hipSuccess;
+}
+
+hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, Layout ly,
+                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t grid,
+                       BatchQueue* d_q, uint32_t stride, hipStream_t s)
+{
+    if (n_items == 0 || grid == 0) return hipSuccess;
+    const void* fn = batchKernel(st, ly, interpolate, unroll);
+    void* args[] = { const_cast<BatchItem**>(&d_items), &n_items, &d_q, &stride };
+    return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, ldsForBlocks(blocks_per_cu), s);
 }
 
 } // namespace ctlk

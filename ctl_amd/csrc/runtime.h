@@ -74,7 +74,7 @@ ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings
                                hipStream_t stream, EffectiveVolume& ev);
 
 ctlproj_status validateSettings(const ctlproj_settings& s);
-ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, uint32_t channels,
-                           uint32_t rows);
+ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly,
+                           uint32_t channels, uint32_t rows);
 
 } // namespace ctlrt

@@ -317,6 +317,7 @@ class Settings:
     interpolate: bool = True
     tile: tuple | None = None  # (wave_w, wave_h), 0 = automatic
     unroll: int = 0
+    blocks_per_cu: int = 0  # cap on resident workgroups per CU (0 = automatic)
 
     def _c(self) -> _lib.Settings:
         s = _lib.Settings()
@@ -327,6 +328,7 @@ class Settings:
         if self.tile:
             s.tile_w, s.tile_h = self.tile
         s.unroll = self.unroll
+        s.blocks_per_cu = self.blocks_per_cu
         return s
 
 
@@ -402,6 +404,13 @@ class RayCasterProjector:
         check(lib().ctlproj_projector_last_kernel_ms(self.h, C.byref(ms)))
         return ms.value
 
+    def autotune(self, volume: ResidentVolume) -> float:
+        """Picks the fastest launch shape for this workload class (see ctlproj_projector_autotune)."""
+        self._push_settings()
+        ms = C.c_float()
+        check(lib().ctlproj_projector_autotune(self.h, volume.h, C.byref(ms)))
+        return ms.value
+
     def count_samples(self, volume: ResidentVolume) -> int:
         self._push_settings()
         n = C.c_uint64()
@@ -429,6 +438,78 @@ class RayCasterProjector:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many projections (own volume, geometry, detector) in one persistent, graph-captured launch.
+
+    Outputs are device arrays owned by the caller (e.g. torch tensors): pass their data_ptr().
+    """
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.h = C.c_void_p()
+        check(lib().ctlproj_batch_create(self.ctx.h, C.byref(self.h)))
+        self._keep = []
+
+    def add(self, volume: ResidentVolume, pmats: np.ndarray, channels: int, rows: int,
+            out_ptr: int, settings: Settings | None = None) -> int:
+        pm = np.ascontiguousarray(pmats, np.float64)
+        if pm.ndim == 3:
+            pm = pm[:, None]
+        V, M = pm.shape[:2]
+        s = (settings or Settings())._c()
+        idx = C.c_uint32()
+        check(lib().ctlproj_batch_add(self.h, volume.h, _lib.dptr(pm), V, M, channels, rows,
+                                      C.byref(s), C.c_void_p(out_ptr), V * M * rows * channels,
+                                      C.byref(idx)))
+        self._keep.append((volume, pm))
+        return idx.value
+
+    def finalize(self, autotune: bool = True) -> None:
+        check(lib().ctlproj_batch_finalize(self.h, 1 if autotune else 0))
+
+    def launch(self) -> None:
+        check(lib().ctlproj_batch_launch(self.h))
+
+    def synchronize(self) -> None:
+        check(lib().ctlproj_batch_synchronize(self.h))
+
+    def last_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().ctlproj_batch_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def info(self) -> dict:
+        v = [C.c_uint32() for _ in range(4)]
+        t = C.c_uint64()
+        check(lib().ctlproj_batch_info(self.h, *[C.byref(x) for x in v], C.byref(t)))
+        return dict(tile_w=v[0].value, tile_h=v[1].value, unroll=v[2].value, grid=v[3].value,
+                    tiles=t.value)
+
+    def count_samples(self) -> int:
+        n = C.c_uint64()
+        check(lib().ctlproj_batch_count_samples(self.h, C.byref(n)))
+        return n.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().ctlproj_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_table() -> str:
+    n = C.c_size_t()
+    check(lib().ctlproj_autotune_table(None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    check(lib().ctlproj_autotune_table(buf, n.value, None))
+    return buf.value.decode()
 
 
 def ellipsoid_phantom(dims, voxel_mm, n_ellipsoids, seed, semi_axes=(4.0, 24.0),

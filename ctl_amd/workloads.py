@@ -89,3 +89,64 @@ def helical(views: int = 2000, dims=(512, 512, 256)) -> Workload:
     vol = ellipsoid_phantom(dims, vox, 40, 3000, mu=(0.0, 0.04))
     return Workload(f"helical: {dims[0]}x{dims[1]}x{dims[2]}, {views} views, 40x16x64, step 0.25 vox",
                     vol, vox, (0.0, 0.0, 0.0), helical_geometry(views), 16, 64, 0.25)
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def carm_arc_poses(rng: np.random.Generator, n_views: int, sid: float = 750.0,
+                   span_deg=(30.0, 360.0)):
+    """A C-arm pose trajectory: views on a circular arc about a random axis through the
+    isocentre (random start rotation, random arc span); source at distance `sid`."""
+    R0 = random_rotation(rng)
+    span = np.deg2rad(rng.uniform(*span_deg))
+    rots, pos = [], []
+    for v in range(n_views):
+        a = span * v / max(1, n_views - 1)
+        c, s = np.cos(a), np.sin(a)
+        Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]])
+        R = R0 @ Rz @ _FIXED
+        rots.append(R)
+        pos.append(R @ np.array([0.0, 0.0, -sid]))
+    return np.array(pos), np.array(rots)
+
+
+# source frame of the C-arm protocols: Rz(pi/2) * Rx(-pi/2) (trajectories.cpp:187-188)
+_FIXED = (np.array([[0, -1.0, 0], [1.0, 0, 0], [0, 0, 1.0]]) @
+          np.array([[1.0, 0, 0], [0, 0, 1.0], [0, -1.0, 0]]))
+
+
+@dataclass
+class BatchInstance:
+    volume: np.ndarray
+    voxel_mm: tuple
+    pmats: np.ndarray
+    channels: int
+    rows: int
+
+
+def batched(n: int = 64, seed: int = 4000, edge=(64, 256), views=(1, 180), panel=(256, 768),
+            fov_mm: float = 128.0, ray_sampling: float = 0.2):
+    """Uneven instances: cubic phantoms of edge U{64..256} covering 128 mm, each with its own
+    random C-arm arc of U{1..180} views and a square flat panel of U{256..768} pixels whose
+    field of view at the isocentre is 1.2 x 128 mm (span 1200 mm, SID 750 mm)."""
+    from .projector import AcquisitionSetup as _AS, CTSystem as _CS
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        N = int(rng.integers(edge[0], edge[1] + 1))
+        V = int(rng.integers(views[0], views[1] + 1))
+        P = int(rng.integers(panel[0], panel[1] + 1))
+        vs = fov_mm / N
+        vol = ellipsoid_phantom((N, N, N), (vs, vs, vs), 12, seed + k)
+        pitch = 1.2 * fov_mm * (1200.0 / 750.0) / P
+        pos, rot = carm_arc_poses(rng, V)
+        setup = _AS(_CS(CarmGantry(1200.0), FlatPanelDetector((P, P), (pitch, pitch))))
+        setup.setCarmPoses(pos, rot.reshape(V, 9))
+        out.append(BatchInstance(vol, (vs, vs, vs), GeometryEncoder.encodeFullGeometry(setup), P, P))
+    return out, ray_sampling

@@ -56,6 +56,7 @@ typedef struct ctlproj_settings {
     uint32_t tile_w;   /* pixels per wavefront tile along channels (0 = auto) */
     uint32_t tile_h;   /* pixels per wavefront tile along rows (0 = auto) */
     uint32_t unroll;   /* ray-step unroll of the sampling loop (0 = auto) */
+    uint32_t blocks_per_cu; /* cap on resident workgroups per CU, 1..8 (0 = auto) */
 } ctlproj_settings;
 
 typedef struct ctlproj_context ctlproj_context;
@@ -127,6 +128,39 @@ ctlproj_status ctlproj_projector_last_kernel_ms(ctlproj_projector* p, float* ms)
  * (sum over rays of floor(lambda_max) + 2 - floor(lambda_min)), counted on the device. */
 ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlproj_volume* vol,
                                                uint64_t* n_samples);
+
+/* Measures the candidate launch shapes (wavefront pixel tile 8x8 / 16x4 / 4x16, ray-step unroll
+ * 2 / 4) on up to 24 of the configured views and remembers the fastest for this workload class
+ * (storage, layout, mode, detector size); later projections with tile/unroll 0 use it.
+ * No counterpart in the reference (it has no launch-shape choice). */
+ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_volume* vol,
+                                         float* best_ms);
+/* The autotuner's table as JSON text (needed = length + 1). */
+ctlproj_status ctlproj_autotune_table(char* buf, size_t cap, size_t* needed);
+
+/* ---- batched projection (no counterpart in the reference) ---------------------------------
+ * Many projections of their own volume, geometry and detector size run in one persistent kernel
+ * captured in a hipGraph: ctlproj_batch_add each entry (device output of views*modules*rows*
+ * channels floats, owned by the caller), ctlproj_batch_finalize once (uploads geometry, optional
+ * autotuning on a sample of the batch's tiles, graph capture), then ctlproj_batch_launch as often
+ * as needed. One batch holds one storage type and one interpolation mode. The callers that
+ * multiply project() calls in the reference (projectComposite, abstractprojector.h:99-112;
+ * SpectralEffectsExtension, spectraleffectsextension.cpp:385-395; ArealFocalSpotExtension,
+ * arealfocalspotextension.cpp:104-150) map onto it. */
+typedef struct ctlproj_batch ctlproj_batch;
+ctlproj_status ctlproj_batch_create(ctlproj_context* ctx, ctlproj_batch** out);
+ctlproj_status ctlproj_batch_destroy(ctlproj_batch* b);
+ctlproj_status ctlproj_batch_add(ctlproj_batch* b, const ctlproj_volume* vol, const double* pmats,
+                                 uint32_t views, uint32_t modules, uint32_t channels, uint32_t rows,
+                                 const ctlproj_settings* settings, float* device_out,
+                                 size_t out_count, uint32_t* index);
+ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune);
+ctlproj_status ctlproj_batch_launch(ctlproj_batch* b);
+ctlproj_status ctlproj_batch_synchronize(ctlproj_batch* b);
+ctlproj_status ctlproj_batch_last_ms(ctlproj_batch* b, float* ms);
+ctlproj_status ctlproj_batch_info(const ctlproj_batch* b, uint32_t* tile_w, uint32_t* tile_h,
+                                  uint32_t* unroll, uint32_t* grid, uint64_t* tiles);
+ctlproj_status ctlproj_batch_count_samples(ctlproj_batch* b, uint64_t* n_samples);
 
 /* ---- geometry front end (GeometryEncoder, geometryencoder.cpp:83-135) -------------------- */
 

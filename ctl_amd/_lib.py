@@ -19,6 +19,7 @@ ERR_NOT_CONFIGURED = 5
 ERR_NO_DATA = 6
 ERR_OVER_BUDGET = 7
 ERR_INJECTED = 8
+ERR_TIMEOUT = 9
 
 F32, F16, BF16 = 0, 1, 2
 CARM, TUBULAR = 0, 1
@@ -64,6 +65,13 @@ class Gantry(C.Structure):
 
 class Trajectory(C.Structure):
     _fields_ = [("type", C.c_int32), ("p", C.c_double * 6)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "rejected", "views",
+                                          "launches")] + \
+               [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "p50_single_ms",
+                                          "p99_single_ms", "p50_multi_ms", "p99_multi_ms", "span_s")]
 
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_int, C.c_void_p)
@@ -118,6 +126,14 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_batch_last_ms": (_i, [_vp, _f32p]),
     "ctlproj_batch_info": (_i, [_vp, _u32p, _u32p, _u32p, _u32p, _u64p]),
     "ctlproj_batch_count_samples": (_i, [_vp, _u64p]),
+    "ctlproj_server_create": (_i, [_vp, _sz, C.c_uint64, _pp]),
+    "ctlproj_server_destroy": (_i, [_vp]),
+    "ctlproj_server_submit": (_i, [_vp, _vp, _f64p, _u32, _u32, _u32, _u32, C.POINTER(Settings), _vp, _sz,
+                                   _u64p]),
+    "ctlproj_server_wait": (_i, [_vp, C.c_uint64, C.c_double, C.POINTER(C.c_double)]),
+    "ctlproj_server_drain": (_i, [_vp]),
+    "ctlproj_server_get_stats": (_i, [_vp, C.POINTER(ServerStats)]),
+    "ctlproj_server_reset_stats": (_i, [_vp]),
 }
 
 _lib: C.CDLL | None = None

@@ -504,6 +504,80 @@ class Batch:
             pass
 
 
+class Server:
+    """Request loop over resident volumes (ctlproj_server_*): admission by HBM footprint,
+    continuous batching across volumes and detector sizes, p50/p99 latency."""
+
+    def __init__(self, hbm_budget: int, max_launch_pixels: int = 0, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.h = C.c_void_p()
+        check(lib().ctlproj_server_create(self.ctx.h, int(hbm_budget), int(max_launch_pixels),
+                                          C.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, volume: ResidentVolume, pmats: np.ndarray, channels: int, rows: int,
+               out: np.ndarray, settings: Settings | None = None) -> int:
+        """Queues a projection into `out` (float32 host array of views*modules*rows*channels,
+        preferably pinned); returns the request id."""
+        pm = np.ascontiguousarray(pmats, np.float64)
+        if pm.ndim == 3:
+            pm = pm[:, None]
+        V, M = pm.shape[:2]
+        s = (settings or Settings())._c()
+        rid = C.c_uint64()
+        check(lib().ctlproj_server_submit(self.h, volume.h, _lib.dptr(pm), V, M, channels, rows,
+                                          C.byref(s), C.c_void_p(out.ctypes.data), out.size,
+                                          C.byref(rid)))
+        self._keep[rid.value] = (volume, out)
+        return rid.value
+
+    def submit_ptr(self, volume: ResidentVolume, pmats: np.ndarray, channels: int, rows: int,
+                   out_ptr: int, count: int, settings: Settings | None = None) -> int:
+        pm = np.ascontiguousarray(pmats, np.float64)
+        if pm.ndim == 3:
+            pm = pm[:, None]
+        V, M = pm.shape[:2]
+        s = (settings or Settings())._c()
+        rid = C.c_uint64()
+        check(lib().ctlproj_server_submit(self.h, volume.h, _lib.dptr(pm), V, M, channels, rows,
+                                          C.byref(s), C.c_void_p(out_ptr), count, C.byref(rid)))
+        self._keep[rid.value] = (volume,)
+        return rid.value
+
+    def wait(self, rid: int, timeout: float = -1.0) -> float:
+        ms = C.c_double()
+        try:
+            check(lib().ctlproj_server_wait(self.h, rid, timeout, C.byref(ms)))
+        finally:
+            if timeout < 0:
+                self._keep.pop(rid, None)
+        return ms.value
+
+    def drain(self) -> None:
+        check(lib().ctlproj_server_drain(self.h))
+        self._keep.clear()
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        check(lib().ctlproj_server_get_stats(self.h, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in _lib.ServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        check(lib().ctlproj_server_reset_stats(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            lib().ctlproj_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def autotune_table() -> str:
     n = C.c_size_t()
     check(lib().ctlproj_autotune_table(None, 0, C.byref(n)))

@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .projector import (DEG, AcquisitionSetup, CarmGantry, CTSystem, CylindricalDetector,
+from .projector import (DEG, AcquisitionSetup, CarmGantry, CTSystem, CylindricalDetector,  # noqa: F401
                         FlatPanelDetector, GeometryEncoder, HelicalTrajectory, ShortScanTrajectory,
                         TubularGantry, ellipsoid_phantom)
 

@@ -36,7 +36,8 @@ typedef enum ctlproj_status {
     CTLPROJ_ERR_NOT_CONFIGURED = 5,   /* project() before configure() */
     CTLPROJ_ERR_NO_DATA = 6,          /* volume without data (reference returns 0 views) */
     CTLPROJ_ERR_OVER_BUDGET = 7,      /* a request larger than the server's HBM budget */
-    CTLPROJ_ERR_INJECTED = 8          /* failure forced by ctlproj_debug_inject_failure */
+    CTLPROJ_ERR_INJECTED = 8,         /* failure forced by ctlproj_debug_inject_failure */
+    CTLPROJ_ERR_TIMEOUT = 9           /* a wait ended before the request finished */
 } ctlproj_status;
 
 typedef enum ctlproj_dtype {
@@ -161,6 +162,48 @@ ctlproj_status ctlproj_batch_last_ms(ctlproj_batch* b, float* ms);
 ctlproj_status ctlproj_batch_info(const ctlproj_batch* b, uint32_t* tile_w, uint32_t* tile_h,
                                   uint32_t* unroll, uint32_t* grid, uint64_t* tiles);
 ctlproj_status ctlproj_batch_count_samples(ctlproj_batch* b, uint64_t* n_samples);
+
+/* ---- serving (no counterpart in the reference) ---------------------------------------------
+ * A request loop over resident volumes: requests are admitted when their device footprint
+ * (output + geometry) fits into the server's HBM arena, split into chunks of views, and
+ * batched across volumes and detector sizes into persistent-kernel launches on two alternating
+ * streams, so requests that arrive while others run join the next launch and launches overlap.
+ * Each result is copied to the caller's host array (pinned memory makes the copy asynchronous);
+ * latency is measured from submit to the result being in host memory. Volumes must outlive the
+ * requests that use them. Calls may come from any thread. */
+typedef struct ctlproj_server ctlproj_server;
+
+typedef struct ctlproj_server_stats {
+    uint64_t submitted, completed, failed, rejected;
+    uint64_t views;      /* views of completed requests */
+    uint64_t launches;   /* persistent-kernel launches */
+    double p50_ms, p99_ms, mean_ms, max_ms;          /* all completed requests */
+    double p50_single_ms, p99_single_ms;             /* one-view requests */
+    double p50_multi_ms, p99_multi_ms;               /* requests of more than one view */
+    double span_s;       /* first submit to last completion */
+} ctlproj_server_stats;
+
+/* hbm_budget: bytes of the arena (outputs + geometry of admitted requests).
+ * max_launch_pixels: pixels (views x modules x rows x channels) per launch, 0 = 64 Mi. */
+ctlproj_status ctlproj_server_create(ctlproj_context* ctx, size_t hbm_budget,
+                                     uint64_t max_launch_pixels, ctlproj_server** out);
+/* Waits for every submitted request, then stops the server. */
+ctlproj_status ctlproj_server_destroy(ctlproj_server* s);
+/* Queues a projection; returns at once. A request larger than the whole arena is rejected with
+ * CTLPROJ_ERR_OVER_BUDGET. Storage type and interpolation mode may differ between requests. */
+ctlproj_status ctlproj_server_submit(ctlproj_server* s, const ctlproj_volume* vol,
+                                     const double* pmats, uint32_t views, uint32_t modules,
+                                     uint32_t channels, uint32_t rows,
+                                     const ctlproj_settings* settings, float* host_out,
+                                     size_t out_count, uint64_t* request_id);
+/* Waits for one request (timeout_s < 0: no limit); returns its status (CTLPROJ_ERR_TIMEOUT if
+ * it is still running). latency_ms may be NULL. */
+ctlproj_status ctlproj_server_wait(ctlproj_server* s, uint64_t request_id, double timeout_s,
+                                   double* latency_ms);
+/* Waits for every request submitted so far. */
+ctlproj_status ctlproj_server_drain(ctlproj_server* s);
+ctlproj_status ctlproj_server_get_stats(ctlproj_server* s, ctlproj_server_stats* out);
+ctlproj_status ctlproj_server_reset_stats(ctlproj_server* s);
 
 /* ---- geometry front end (GeometryEncoder, geometryencoder.cpp:83-135) -------------------- */
 

@@ -1,8 +1,13 @@
-"""Benchmark: forward projection of a 256^3 ellipsoid phantom, 360 views, 1024x768 flat panel.
+"""Benchmark of the forward projector on one MI355X.
 
-One step = one full 360-view projection with the volume resident in HBM and the 1.13 GB output
-written to HBM. Prints one JSON line (see the keys below). Run: python bench.py [--steps K]
-[--warmup W]. Only one GPU is used; --gpus N > 1 exits with a message.
+Headline (`value`): the FP32 drop-in on a 256^3 ellipsoid phantom, 360 views, 1024x768 flat
+panel, ray step 0.2 voxel. One step = one full 360-view projection with the volume resident in
+HBM and the 1.13 GB output written to HBM. In the same run: FP16/BF16 storage, the helical
+workload (40-module curved detector), 64 uneven instances in one persistent graph-captured
+launch, and a served request stream (p50/p99 latency); errors against the CPU oracle (ref32 and
+ref64) and the oracle timed on the host cores as the CPU baseline.
+
+python bench.py [--steps K] [--warmup W] [--quick]   (one GPU; --gpus N > 1 exits)
 """
 from __future__ import annotations
 
@@ -15,8 +20,245 @@ import time
 import numpy as np
 
 HBM_PEAK_GBPS = 8000.0
-# gfx950 vector L1: 256 CUs x 2.4 GHz x 128 B/clk
+# gfx950 vector L1 data return: 256 CUs x 2.4 GHz x 128 B/clk
 L1_PEAK_GBPS = 256 * 2.4 * 128.0
+
+# HBM traffic per launch of the 360-view FP32 projection, from the rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE passes in profiles/raycast_f32_256cube_360views_1024x768.md, in bytes
+# (KiB x 1024, uncalibrated).
+TRAFFIC_F32_BYTES = (4.5377e7 + 1.299e6) * 1024
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_projection(p, vol, out, steps, warmup):
+    n_out = out.numel()
+    for _ in range(warmup):
+        p.project_device(vol, out.data_ptr(), n_out)
+    p.synchronize()
+    kern = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        p.project_device(vol, out.data_ptr(), n_out)
+        kern.append(p.last_kernel_ms())  # waits for this step's kernel
+    p.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, float(np.mean(kern))
+
+
+def bench_real(args, torch, W, ctl, oracle):
+    w = W.real(views=360)
+    log(f"workload: {w.name}")
+    p = ctl.RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((w.pmats, w.channels, w.rows))
+    out = torch.empty(w.views * w.modules * w.rows * w.channels, dtype=torch.float32, device="cuda")
+    res = {}
+    for storage in args.variants.split(","):
+        vol = ctl.ResidentVolume(ctl.VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
+        n_samples = p.count_samples(vol)
+        wall, kms = timed_projection(p, vol, out, args.steps, args.warmup)
+        eb = 4 if storage == "f32" else 2
+        res[storage] = dict(ms_per_step=wall * 1e3, kernel_ms=kms, n_samples=n_samples,
+                            gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall,
+                            kernel_gsamples_s=n_samples / (kms * 1e-3) / 1e9,
+                            gather_GBps=n_samples * 8 * eb / (kms * 1e-3) / 1e9)
+        log(f"real {storage}: {json.dumps(res[storage])}")
+        del vol
+    if not args.no_errors:
+        pick = [0, 137]
+        r32, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
+                                w.ray_sampling, views=pick, threads=16)
+        r64, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
+                                w.ray_sampling, views=pick, threads=16, fp64=True)
+        p2 = ctl.RayCasterProjector()
+        p2.settings().raySampling = w.ray_sampling
+        p2.configure((w.pmats[pick], w.channels, w.rows))
+        for storage in res:
+            got = p2.project(ctl.ResidentVolume(ctl.VoxelVolume(w.volume, w.voxel_mm), storage)).data
+            res[storage]["rel_l2_vs_ref64"] = oracle.rel_l2(r64, got)
+            res[storage]["error_views"] = pick
+            if storage == "f32":
+                res[storage]["rel_l2_vs_ref32"] = oracle.rel_l2(r32, got)
+                res[storage]["max_abs_rel_vs_ref32"] = oracle.max_abs_rel(r32, got)
+    # end-to-end reference-style project(): host volume upload, projection, host output
+    pr = ctl.RayCasterProjector()
+    pr.settings().raySampling = w.ray_sampling
+    pr.configure((w.pmats, w.channels, w.rows))
+    t0 = time.perf_counter()
+    pr.project(ctl.VoxelVolume(w.volume, w.voxel_mm, w.offset_mm))
+    res["f32"]["end_to_end_s"] = time.perf_counter() - t0
+    log(f"real f32 end to end (upload + project + 1.13 GB to host): {res['f32']['end_to_end_s']:.3f} s")
+    return w, res
+
+
+def bench_helical(args, torch, W, ctl, oracle):
+    w = W.helical(views=2000)
+    log(f"workload: {w.name}")
+    p = ctl.RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((w.pmats, w.channels, w.rows))
+    out = torch.empty(w.views * w.modules * w.rows * w.channels, dtype=torch.float32, device="cuda")
+    res = {}
+    for storage in ("f32", "f16"):
+        vol = ctl.ResidentVolume(ctl.VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
+        n_samples = p.count_samples(vol)
+        wall, kms = timed_projection(p, vol, out, max(1, args.steps // 2), 1)
+        res[storage] = dict(ms_per_step=wall * 1e3, kernel_ms=kms, n_samples=n_samples,
+                            gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall)
+        log(f"helical {storage}: {json.dumps(res[storage])}")
+        if not args.no_errors:
+            pick = [0, 1234]
+            r64, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
+                                    w.ray_sampling, views=pick, threads=16, fp64=True)
+            p2 = ctl.RayCasterProjector()
+            p2.settings().raySampling = w.ray_sampling
+            p2.configure((w.pmats[pick], w.channels, w.rows))
+            got = p2.project(vol).data
+            res[storage]["rel_l2_vs_ref64"] = oracle.rel_l2(r64, got)
+            if storage == "f32":
+                r32, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels,
+                                        w.rows, w.ray_sampling, views=pick, threads=16)
+                res[storage]["rel_l2_vs_ref32"] = oracle.rel_l2(r32, got)
+        del vol
+    return res
+
+
+def bench_batched(args, torch, W, ctl):
+    items, rs = W.batched(n=64)
+    views = sum(i.pmats.shape[0] for i in items)
+    log(f"batched: 64 instances, {views} views")
+    outs = [torch.empty(i.pmats.shape[0] * i.rows * i.channels, dtype=torch.float32, device="cuda")
+            for i in items]
+    res = {"instances": 64, "views": views}
+    for storage in ("f32", "f16"):
+        vols = [ctl.ResidentVolume(ctl.VoxelVolume(i.volume, i.voxel_mm), storage) for i in items]
+        b = ctl.Batch()
+        for i, v, o in zip(items, vols, outs):
+            b.add(v, i.pmats, i.channels, i.rows, o.data_ptr(), ctl.Settings(raySampling=rs))
+        t0 = time.perf_counter()
+        b.finalize(autotune=True)
+        fin = time.perf_counter() - t0
+        ns = b.count_samples()
+        b.launch()
+        b.synchronize()
+        t0 = time.perf_counter()
+        ms = []
+        for _ in range(max(1, args.steps // 2)):
+            b.launch()
+            ms.append(b.last_ms())
+        wall = (time.perf_counter() - t0) / len(ms)
+        res[storage] = dict(ms_per_launch=wall * 1e3, kernel_ms=float(np.mean(ms)), n_samples=ns,
+                            gsamples_s=ns / wall / 1e9, views_s=views / wall, finalize_s=fin,
+                            shape=b.info())
+        log(f"batched {storage}: {json.dumps(res[storage])}")
+        b.close()
+        del vols
+    # baseline: 64 sequential plain FP32 project() calls (upload, project, output to host)
+    t0 = time.perf_counter()
+    for i in items:
+        p = ctl.RayCasterProjector()
+        p.settings().raySampling = rs
+        p.configure((i.pmats, i.channels, i.rows))
+        p.project(ctl.VoxelVolume(i.volume, i.voxel_mm))
+    seq = time.perf_counter() - t0
+    res["sequential_plain_f32"] = dict(s=seq, views_s=views / seq)
+    res["speedup_f16_batched_vs_sequential_plain_f32"] = res["f16"]["views_s"] / (views / seq)
+    res["speedup_f32_batched_vs_sequential_plain_f32"] = res["f32"]["views_s"] / (views / seq)
+    log(f"batched sequential plain f32: {seq:.3f} s")
+    return res
+
+
+def bench_served(args, torch, W, ctl):
+    """8 resident 256^3 FP16 phantoms; Poisson arrivals; 80% one-view 512x512 DRRs near a
+    reference C-arm pose (+-5 deg, +-10 mm), 20% 180-view 768x768 scans."""
+    rng = np.random.default_rng(5)
+    phantoms = [W.ellipsoid_phantom((256, 256, 256), (0.5, 0.5, 0.5), 20, 5000 + k) for k in range(8)]
+    vols = [ctl.ResidentVolume(ctl.VoxelVolume(d, (0.5, 0.5, 0.5)), "f16") for d in phantoms]
+
+    def drr_pose():
+        R = np.eye(3)
+        for ax, a in enumerate(np.deg2rad(rng.uniform(-5, 5, 3))):
+            c, s = np.cos(a), np.sin(a)
+            M = np.eye(3)
+            i, j = [(1, 2), (0, 2), (0, 1)][ax]
+            M[i, i] = c
+            M[j, j] = c
+            M[i, j] = -s
+            M[j, i] = s
+            R = R @ M
+        rot = R @ W._FIXED
+        pos = rot @ np.array([0, 0, -750.0]) + rng.uniform(-10, 10, 3)
+        setup = ctl.AcquisitionSetup(ctl.CTSystem(ctl.CarmGantry(1200.0),
+                                                  ctl.FlatPanelDetector((512, 512), (0.8, 0.8))))
+        setup.setCarmPoses(pos[None], rot.reshape(1, 9))
+        return ctl.GeometryEncoder.encodeFullGeometry(setup)
+
+    scan_pm = W.carm_circle(180, (768, 768), (0.533, 0.533))
+    drr_out = [torch.empty(512 * 512, dtype=torch.float32).pin_memory() for _ in range(64)]
+    scan_out = [torch.empty(180 * 768 * 768, dtype=torch.float32).pin_memory() for _ in range(4)]
+    st = ctl.Settings(raySampling=0.2)
+    srv = ctl.Server(hbm_budget=16 << 30)
+    rid = srv.submit_ptr(vols[0], scan_pm, 768, 768, scan_out[0].data_ptr(), scan_out[0].numel(), st)
+    srv.wait(rid)
+    res = {"phantoms": "8 x 256^3 FP16 resident", "mix": "80% DRR 512x512, 20% scan 180x768x768"}
+    n = 60 if args.quick else 120
+    for rate in (200, 1000):
+        gaps = rng.exponential(1.0 / rate, n)
+        kinds = rng.uniform(size=n) < 0.8
+        geoms = [drr_pose() if k else None for k in kinds]
+        srv.reset_stats()
+        t = time.perf_counter()
+        for i in range(n):
+            t += gaps[i]
+            while time.perf_counter() < t:
+                pass
+            v = vols[int(rng.integers(8))]
+            if kinds[i]:
+                o = drr_out[i % 64]
+                srv.submit_ptr(v, geoms[i], 512, 512, o.data_ptr(), o.numel(), st)
+            else:
+                o = scan_out[i % 4]
+                srv.submit_ptr(v, scan_pm, 768, 768, o.data_ptr(), o.numel(), st)
+        srv.drain()
+        s = srv.stats()
+        res[f"rate_{rate}"] = dict(requests=n, p50_ms=s["p50_ms"], p99_ms=s["p99_ms"],
+                                   p50_drr_ms=s["p50_single_ms"], p99_drr_ms=s["p99_single_ms"],
+                                   p50_scan_ms=s["p50_multi_ms"], p99_scan_ms=s["p99_multi_ms"],
+                                   views_s=s["views"] / s["span_s"], launches=s["launches"])
+        log(f"served {rate}/s: {json.dumps(res[f'rate_{rate}'])}")
+    srv.close()
+    # plain per-request FP32 path: upload the FP32 volume, project, copy to host, per request
+    p = ctl.RayCasterProjector()
+    p.settings().raySampling = 0.2
+    t0 = time.perf_counter()
+    nv = 0
+    for k in range(10):
+        pm = drr_pose() if k < 8 else scan_pm
+        p.configure((pm, 512 if k < 8 else 768, 512 if k < 8 else 768))
+        p.project(ctl.VoxelVolume(phantoms[k % 8], (0.5, 0.5, 0.5)))
+        nv += pm.shape[0]
+    dt = time.perf_counter() - t0
+    res["plain_per_request_f32"] = dict(requests=10, views=nv, s=dt, views_s=nv / dt)
+    res["speedup_views_s"] = res["rate_1000"]["views_s"] / (nv / dt)
+    log(f"served plain per-request f32: {nv / dt:.1f} views/s")
+    return res
+
+
+def cpu_baseline(w, oracle):
+    threads = min(16, os.cpu_count() or 1)
+    sub = list(range(0, 360, 45))  # 8 evenly strided views
+    t0 = time.perf_counter()
+    _, n = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
+                          w.ray_sampling, views=sub, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt / 1e9, "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "sample": f"ref32 oracle (C restatement of the reference kernel, pthreads) on views "
+                      f"{sub[0]}..{sub[-1]} step 45 (8 of 360 views) of the same workload, "
+                      f"{n} samples in {dt:.1f} s",
+            "views_per_s": len(sub) / dt}
 
 
 def main() -> int:
@@ -24,110 +266,64 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variants", default="f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-errors", action="store_true", help="skip the oracle error statements")
-    ap.add_argument("--variants", default="f32,f16,bf16")
+    ap.add_argument("--only-real", action="store_true", help="only the 256^3 / 360-view workload")
+    ap.add_argument("--quick", action="store_true", help="fewer served requests")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the projector shards over views with no collective; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported here. The "
+              "projector shards over views with no collective (see DESIGN.md section 7).",
+              file=sys.stderr)
         return 2
 
     import torch
 
-    from ctl_amd import RayCasterProjector, ResidentVolume, VoxelVolume
+    import ctl_amd as ctl
     from ctl_amd import workloads as W
-
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
-    w = W.real(views=360)
-    log(f"workload: {w.name}")
-    p = RayCasterProjector()
-    p.settings().raySampling = w.ray_sampling
-    p.configure((w.pmats, w.channels, w.rows))
-    n_out = w.views * w.modules * w.rows * w.channels
-    out = torch.empty(n_out, dtype=torch.float32, device="cuda")
-
-    results = {}
-    for storage in args.variants.split(","):
-        vol = ResidentVolume(VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
-        n_samples = p.count_samples(vol)
-        for _ in range(args.warmup):
-            p.project_device(vol, out.data_ptr(), n_out)
-        p.synchronize()
-        torch.cuda.synchronize()
-        kern = []
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            p.project_device(vol, out.data_ptr(), n_out)
-            kern.append(p.last_kernel_ms())  # waits for this step's kernel
-        p.synchronize()
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / args.steps
-        kms = float(np.mean(kern))
-        eb = 4 if storage == "f32" else 2
-        results[storage] = dict(
-            ms_per_step=wall * 1e3, kernel_ms=kms, n_samples=n_samples,
-            gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall,
-            kernel_gsamples_s=n_samples / (kms * 1e-3) / 1e9,
-            gather_GBps=n_samples * 8 * eb / (kms * 1e-3) / 1e9)
-        log(f"{storage}: {json.dumps(results[storage])}")
-        del vol
-
-    # error statements against the oracle on two views (ref32 for FP32, ref64 for all)
-    if not args.no_errors:
+    oracle = None
+    if not (args.no_cpu and args.no_errors):
         import oracle
-        pick = [0, 137]
-        r32, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
-                                w.ray_sampling, views=pick, threads=16)
-        r64, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
-                                w.ray_sampling, views=pick, threads=16, fp64=True)
-        p2 = RayCasterProjector()
-        p2.settings().raySampling = w.ray_sampling
-        p2.configure((w.pmats[pick], w.channels, w.rows))
-        for storage in results:
-            vol = ResidentVolume(VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
-            got = p2.project(vol).data
-            results[storage]["rel_l2_vs_ref64"] = oracle.rel_l2(r64, got)
-            if storage == "f32":
-                results[storage]["rel_l2_vs_ref32"] = oracle.rel_l2(r32, got)
-                results[storage]["max_abs_rel_vs_ref32"] = oracle.max_abs_rel(r32, got)
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        threads = min(16, os.cpu_count() or 1)
-        sub = list(range(0, 360, 45))  # 8 evenly strided views
-        t0 = time.perf_counter()
-        _, n = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
-                              w.ray_sampling, views=sub, threads=threads)
-        dt = time.perf_counter() - t0
-        cpu = {"value": n / dt / 1e9, "unit": "Gsamples/s", "cores": threads, "kind": "port",
-               "sample": f"ref32 oracle on views {sub[0]}..{sub[-1]} step 45 (8 of 360 views), "
-                         f"{n} samples in {dt:.1f} s; views/s {len(sub) / dt:.3f}",
-               "views_per_s": len(sub) / dt}
+    w, real = bench_real(args, torch, W, ctl, oracle)
+    extra = {}
+    if not args.only_real:
+        extra["helical"] = bench_helical(args, torch, W, ctl, oracle)
+        extra["batched"] = bench_batched(args, torch, W, ctl)
+        extra["served"] = bench_served(args, torch, W, ctl)
+    cpu = None if args.no_cpu else cpu_baseline(w, oracle)
+    if cpu:
         log(f"cpu baseline: {cpu}")
 
-    f32 = results.get("f32") or next(iter(results.values()))
+    f32 = real["f32"]
     achieved = f32["gather_GBps"]
     line = {
         "metric": "Gsamples/s & views/s at 256³/360 views vs plain FP32; served p50/p99 latency",
         "value": f32["gsamples_s"], "unit": "Gsamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": f32["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": w.name, "volume": "256^3 x 0.5 mm, 20 seeded ellipsoids",
-                   "views": 360, "detector": "1024x768 @ 0.4 mm", "ray_step_vox": 0.2,
-                   "parallelism": "single GPU"},
+        "config": {"workload": w.name, "volume": "256^3 x 0.5 mm, 20 seeded ellipsoids, FP32",
+                   "views": 360, "detector": "1024x768 @ 0.4 mm, C-arm 1200 mm / SID 750 mm",
+                   "ray_step_vox": 0.2, "parallelism": "single GPU"},
         "views_per_s": f32["views_s"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": TRAFFIC_F32_BYTES,
                      "algorithmic_bytes_per_sample": 32, "samples_per_launch": f32["n_samples"],
                      "kernel_ms": f32["kernel_ms"],
+                     "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample); they "
+                             "are served from L1/L2, traffic = HBM bytes per launch",
                      "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS},
         "cpu_baseline": cpu,
-        "variants": results,
+        "variants": real,
     }
+    line.update(extra)
+    if "f16" in real:
+        line["f16_vs_f32_views_s"] = real["f16"]["views_s"] / f32["views_s"]
+    if "served" in extra:
+        sv = extra["served"]
+        line["served_p50_ms"] = sv["rate_200"]["p50_ms"]
+        line["served_p99_ms"] = sv["rate_200"]["p99_ms"]
     print(json.dumps(line), flush=True)
     return 0
 

@@ -20,8 +20,9 @@ import time
 import numpy as np
 
 HBM_PEAK_GBPS = 8000.0
-# gfx950 vector L1 data return: 256 CUs x 2.4 GHz x 128 B/clk
-L1_PEAK_GBPS = 256 * 2.4 * 128.0
+# Vector L1 to register data path of gfx950 for 16-byte loads: 256 CUs x 2.4 GHz x 64 B/clk. The
+# stream probe (coalesced 16-byte loads of an L1-resident window) reaches 98% of it.
+L1_PEAK_GBPS = 256 * 2.4 * 64.0
 
 # HBM traffic per launch of the 360-view FP32 projection, from the rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE passes in profiles/raycast_f32_256cube_360views_1024x768.md, in bytes
@@ -65,6 +66,14 @@ def bench_real(args, torch, W, ctl, oracle):
                             gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall,
                             kernel_gsamples_s=n_samples / (kms * 1e-3) / 1e9,
                             gather_GBps=n_samples * 8 * eb / (kms * 1e-3) / 1e9)
+        # gather ceiling of this access pattern: the same loads without the interpolation
+        pg = [p.gather_probe(vol, "gather", out.data_ptr(), out.numel()) for _ in range(4)][1:]
+        ceil = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in pg]))
+        res[storage]["probe_gather_GBps"] = ceil
+        res[storage]["frac_of_probe_gather"] = res[storage]["gather_GBps"] / ceil
+        if storage == "f32":
+            ps = [p.gather_probe(vol, "stream", out.data_ptr(), out.numel()) for _ in range(4)][1:]
+            res[storage]["probe_l1_stream_GBps"] = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in ps]))
         log(f"real {storage}: {json.dumps(res[storage])}")
         del vol
     if not args.no_errors:
@@ -254,7 +263,14 @@ def cpu_baseline(w, oracle):
     _, n = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
                           w.ray_sampling, views=sub, threads=threads)
     dt = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": n / dt / 1e9, "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(), "cpu_model": model,
             "sample": f"ref32 oracle (C restatement of the reference kernel, pthreads) on views "
                       f"{sub[0]}..{sub[-1]} step 45 (8 of 360 views) of the same workload, "
                       f"{n} samples in {dt:.1f} s",
@@ -312,8 +328,13 @@ def main() -> int:
                      "algorithmic_bytes_per_sample": 32, "samples_per_launch": f32["n_samples"],
                      "kernel_ms": f32["kernel_ms"],
                      "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample); they "
-                             "are served from L1/L2, traffic = HBM bytes per launch",
-                     "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS},
+                             "are served from L1/L2, traffic = HBM bytes per launch. The on-chip "
+                             "peak is the vector L1 data path (l1_gather_peak, 64 B/clk/CU); "
+                             "probe_gather_GBps = the same loads without interpolation",
+                     "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS,
+                     "probe_gather_GBps": f32.get("probe_gather_GBps"),
+                     "frac_of_probe_gather": f32.get("frac_of_probe_gather"),
+                     "probe_l1_stream_GBps": f32.get("probe_l1_stream_GBps")},
         "cpu_baseline": cpu,
         "variants": real,
     }

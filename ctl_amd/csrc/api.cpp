@@ -724,6 +724,62 @@ ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_vo
     return CTLPROJ_OK;
 }
 
+// ---- gather-ceiling probes (diagnostics) -----------------------------------------------------
+
+ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_volume* vol, int mode,
+                                         float* device_out, size_t out_count, float* ms, double* bytes)
+{
+    if (!ms || !bytes) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (mode != CTLPROJ_PROBE_GATHER && mode != CTLPROJ_PROBE_STREAM)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unknown probe mode");
+    ctlproj_status st = CTLPROJ_OK;
+    hipError_t e = hipSuccess;
+    if (mode == CTLPROJ_PROBE_GATHER) {
+        if ((st = checkProject(p, vol, device_out, out_count)) != CTLPROJ_OK) return st;
+        if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+        if (p->V == 0) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "no views to probe");
+        if (!p->s.interpolate || p->s.rays_per_pixel[0] != 1 || p->s.rays_per_pixel[1] != 1 ||
+            p->s.volume_upsampling != 1 || vol->view.layout != ctlk::Layout::Interleaved)
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT,
+                        "the gather probe needs trilinear, one ray per pixel, no upsampling and an "
+                        "interleaved volume");
+        EffectiveVolume ev;
+        if ((st = effectiveVolume(vol, p->s, p->stream, ev)) != CTLPROJ_OK) return st;
+        if ((st = prepareGeometry(p, ev)) != CTLPROJ_OK) return st;
+        const ctlk::RayParams rp = rayParams(p, p->V);
+        unsigned long long n = 0;
+        e = hipMemsetAsync(p->d_count, 0, sizeof(unsigned long long), p->stream);
+        if (e == hipSuccess) e = ctlk::launchCount(ev.geom, p->d_geom, rp, p->d_count, p->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&n, p->d_count, sizeof n, hipMemcpyDeviceToHost, p->stream);
+        if (e == hipSuccess) e = hipEventRecord(p->ev_start, p->stream);
+        if (e == hipSuccess) e = ctlk::launchProbeGather(ev.view, ev.geom, p->d_geom, rp, device_out, p->stream);
+        if (e == hipSuccess) e = hipEventRecord(p->ev_stop, p->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(p->ev_stop);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, p->ev_start, p->ev_stop);
+        if (e != hipSuccess) return hipFail(e, "gather probe");
+        *bytes = double(n) * (ev.view.storage == ctlk::Storage::F32 ? 32.0 : 16.0);
+        return CTLPROJ_OK;
+    }
+    if (!p || !vol || !vol->d_data || !device_out) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (vol->ctx != p->ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume belongs to another context");
+    if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+    int cus = 0;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->ctx->device);
+    if (e != hipSuccess) return hipFail(e, "hipDeviceGetAttribute");
+    const uint32_t grid = uint32_t(cus) * 8, iters = 16384;
+    if (out_count < size_t(grid) * 256)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream probe needs 2048 floats per CU of output");
+    if (vol->view.bytes < 16384) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream probe needs a volume of 16 KiB or more");
+    e = hipEventRecord(p->ev_start, p->stream);
+    if (e == hipSuccess) e = ctlk::launchProbeStream(vol->view.data, grid, iters, device_out, p->stream);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_stop, p->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(p->ev_stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, p->ev_start, p->ev_stop);
+    if (e != hipSuccess) return hipFail(e, "stream probe");
+    *bytes = double(grid) * 256.0 * iters * 16.0;
+    return CTLPROJ_OK;
+}
+
 // ---- geometry --------------------------------------------------------------------------------
 
 static bool toDetector(const ctlproj_detector* d, ctlgeo::Detector& out)

@@ -147,6 +147,15 @@ hipError_t launchFillMean(const double* d_sum, double count, Storage st, Layout 
 // Zero-fill of a float array with a plain kernel (graph-capture safe).
 hipError_t launchZero(float* d, size_t n, hipStream_t s);
 
+// Gather-ceiling probes (diagnostics). launchProbeGather issues the ray caster's texel loads for
+// every ray of the projection (interleaved layout, trilinear, one ray per pixel) without the
+// interpolation; FP16 and BF16 share one load pattern. launchProbeStream runs `grid` workgroups
+// of coalesced 16-byte loads, `iters` (a multiple of 4) per thread, from a 16 KiB window of d_src;
+// d_out receives grid * 256 floats.
+hipError_t launchProbeGather(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
+                             const RayParams& rp, float* d_out, hipStream_t s);
+hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, float* d_out, hipStream_t s);
+
 bool validShape(const LaunchShape& s);
 
 } // namespace ctlk

@@ -355,11 +355,6 @@ __device__ __forceinline__ float sampleAt(__amdgpu_buffer_rsrc_t rs, const Pitch
     return INTERP ? trilinear<ST, LY>(rs, p, qx, qy, qz) : nearest<ST, LY>(rs, p, qx, qy, qz);
 }
 
-// One thread per detector pixel. All rays of a wavefront step through the same sample indices
-// i (the same distance from the source), so the 64 gathers of one load instruction fall into a
-// small patch of the volume and share cache lines. Each ray's own range [i0, i1] (the reference's
-// loop bounds, trimmed to samples that can touch the volume) is honoured with the exec mask in the
-// head and tail of the wave's range; in between every active ray samples, unmasked and unrolled.
 // Processes logical tile b of one projection: one thread per detector pixel. All rays of a
 // wavefront step through the same sample indices i (the same distance from the source), so the
 // 64 gathers of one load instruction fall into a small patch of the volume and share cache lines.
@@ -537,6 +532,105 @@ __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict
     if (mine) atomicAdd(&block_sum, mine);
     __syncthreads();
     if (threadIdx.x == 0 && block_sum) atomicAdd(count, block_sum);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gather-ceiling probes. Diagnostics that measure the ceiling of the gather path; no projection
+// runs through them.
+//
+//   probe_gather_kernel  the ray caster's loads without its interpolation: the same tiles, rays,
+//                        sample ranges, wave-aligned loop and 16-byte texel loads over an
+//                        interleaved volume, with the raw taps folded into an integer sink.
+//   probe_stream_kernel  coalesced 16-byte loads from a 16 KiB window that stays in the vector
+//                        L1: the cache-to-register data path without any gather.
+
+typedef uint32_t probe_u4 __attribute__((ext_vector_type(4)));
+
+template <Storage ST>
+__device__ __forceinline__ uint32_t rawTaps(__amdgpu_buffer_rsrc_t r, const Pitch& p, float fi,
+                                            const float s[3], const float c[3])
+{
+    const float qx = fmaf(fi, s[0], c[0]);
+    const float qy = fmaf(fi, s[1], c[1]);
+    const float qz = fmaf(fi, s[2], c[2]);
+    const uint32_t idx = uint32_t(int(qx)) + __umul24(uint32_t(int(qy)), p.ty) +
+                         __umul24(uint32_t(int(qz)), p.tz);
+    const probe_u4 a = __builtin_bit_cast(probe_u4, __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, 0, 0));
+    uint32_t h = a.x ^ a.y ^ a.z ^ a.w;
+    if constexpr (ST == Storage::F32) { // second row of taps (y+1)
+        const probe_u4 b = __builtin_bit_cast(probe_u4, __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, p.y, 0));
+        h ^= b.x ^ b.y ^ b.z ^ b.w;
+    }
+    return h;
+}
+
+template <Storage ST>
+__global__ __launch_bounds__(kBlock) void probe_gather_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
+                                                              RayParams rp, CastConst kc,
+                                                              float* __restrict__ out)
+{
+    constexpr int U = 4;
+    const uint32_t b = xcdRemap(blockIdx.x, gridDim.x), wave = threadIdx.x >> 6;
+    const uint32_t vm = b / kc.tiles_per_vm;
+    const uint32_t t = b - vm * kc.tiles_per_vm;
+    const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
+    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
+    const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
+    const bool pixel = x < rp.channels && y < rp.rows;
+    const RayGeom g = geom[vm];
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+
+    uint32_t acc = 0;
+    if (g.valid) {
+        Ray ray = setupRay(g, float(x), float(y), kc.inc, kc.vs, kc.nhi, kc.blo); // one ray per pixel
+        if (!pixel) ray.hit = false;
+        tighten(ray, g.c, kc.blo, kc.nhi);
+        const bool act = pixel && ray.i0 <= ray.i1;
+        const int i0 = act ? ray.i0 : INT_MAX, i1 = act ? ray.i1 : INT_MIN;
+        const int lo = waveMin(i0), hi = waveMax(i1);
+        if (lo <= hi) {
+            int a = waveMax(act ? i0 : INT_MIN), bb = waveMin(act ? i1 : INT_MAX);
+            if (a > bb) { a = hi + 1; bb = hi; }
+            const uint32_t span = uint32_t(i1 - i0);
+            for (int i = lo; i < a; ++i)
+                if (act && uint32_t(i - i0) <= span) acc += rawTaps<ST>(rs, kc.pitch, float(i), ray.s, g.c);
+            if (act) {
+                int i = a;
+                float fi = float(a);
+                for (; i + U - 1 <= bb; i += U, fi += float(U)) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) acc += rawTaps<ST>(rs, kc.pitch, fi + float(u), ray.s, g.c);
+                }
+                for (; i <= bb; ++i, fi += 1.0f) acc += rawTaps<ST>(rs, kc.pitch, fi, ray.s, g.c);
+            }
+            for (int i = bb + 1; i <= hi; ++i)
+                if (act && uint32_t(i - i0) <= span) acc += rawTaps<ST>(rs, kc.pitch, float(i), ray.s, g.c);
+        }
+    }
+    if (pixel) out[(size_t(vm) * rp.rows + y) * rp.channels + x] = float(acc & 0xffffu);
+}
+
+// `step` and `mask` are runtime values (4096 and 16383) so that no load address is loop-invariant
+// at compile time and every load is issued.
+__global__ __launch_bounds__(kBlock) void probe_stream_kernel(const void* __restrict__ src, uint32_t iters,
+                                                              uint32_t step, uint32_t mask,
+                                                              float* __restrict__ out)
+{
+    const __amdgpu_buffer_rsrc_t r = makeRsrc(src, mask + 1);
+    uint32_t ofs = threadIdx.x * 16; // a wavefront reads 1 KiB contiguous per load
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < iters; k += 4) {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const probe_u4 a = __builtin_bit_cast(probe_u4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+            acc += a.x ^ a.y ^ a.z ^ a.w;
+            ofs = (ofs + step) & mask;
+        }
+    }
+    out[size_t(blockIdx.x) * kBlock + threadIdx.x] = float(acc & 0xffffu);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -735,6 +829,29 @@ hipError_t launchCastI(const VolumeView& vol, const RayGeom* g, const RayParams&
 }
 
 } // namespace
+
+hipError_t launchProbeGather(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
+                             const RayParams& rp, float* d_out, hipStream_t s)
+{
+    if (vol.layout != Layout::Interleaved || !rp.interpolate || rp.rpp_x != 1 || rp.rpp_y != 1)
+        return hipErrorInvalidValue;
+    const CastConst kc = makeCastConst(&vol, vg, rp, LaunchShape{});
+    const uint64_t blocks = uint64_t(kc.tiles_per_vm) * rp.views * rp.modules;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
+    if (vol.storage == Storage::F32)
+        probe_gather_kernel<Storage::F32><<<uint32_t(blocks), kBlock, 0, s>>>(vol, d_geom, rp, kc, d_out);
+    else
+        probe_gather_kernel<Storage::F16><<<uint32_t(blocks), kBlock, 0, s>>>(vol, d_geom, rp, kc, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, float* d_out, hipStream_t s)
+{
+    if (grid == 0 || iters % 4) return hipErrorInvalidValue;
+    probe_stream_kernel<<<grid, kBlock, 0, s>>>(d_src, iters, 4096u, 16383u, d_out); // 16 KiB window
+    return hipGetLastError();
+}
 
 bool validShape(const LaunchShape& s)
 {

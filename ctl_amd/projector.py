@@ -417,6 +417,17 @@ class RayCasterProjector:
         check(lib().ctlproj_projector_count_samples(self.h, volume.h, C.byref(n)))
         return n.value
 
+    def gather_probe(self, volume: ResidentVolume, mode: str, device_out: int, out_count: int):
+        """Gather-ceiling diagnostic (ctlproj_diag_gather_probe): mode "gather" issues this
+        projection's texel loads without the interpolation, mode "stream" reads a 16 KiB
+        L1-resident window with coalesced 16-byte loads. Returns (milliseconds, bytes loaded)."""
+        self._push_settings()
+        code = {"gather": 0, "stream": 1}[mode]
+        ms, nbytes = C.c_float(), C.c_double()
+        check(lib().ctlproj_diag_gather_probe(self.h, volume.h, code, C.c_void_p(device_out),
+                                              out_count, C.byref(ms), C.byref(nbytes)))
+        return ms.value, nbytes.value
+
     def isLinear(self) -> bool:  # abstractprojector.h:97
         return True
 

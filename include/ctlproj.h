@@ -261,6 +261,21 @@ ctlproj_status ctlproj_phantom_ellipsoids(const uint32_t dims[3], const float vo
                                           float mu_min, float mu_max, float background_mu,
                                           float* out);
 
+/* ---- gather-ceiling probes (diagnostics) -------------------------------------------------
+ * No counterpart in the reference: they measure the ceiling against which the ray caster's gather
+ * rate is judged, and compute no projection.
+ *   CTLPROJ_PROBE_GATHER  the configured projection's texel loads (same rays, sample ranges,
+ *                         wave-aligned loop and 16-byte loads) without the interpolation; needs
+ *                         trilinear, rays_per_pixel 1x1, upsampling 1 and an interleaved volume;
+ *                         device_out as for ctlproj_project_device (its values are meaningless).
+ *                         *bytes = samples x 8 taps x element size.
+ *   CTLPROJ_PROBE_STREAM  coalesced 16-byte loads from a 16 KiB window of the volume that stays in
+ *                         the vector L1; device_out needs 2048 floats per CU. *bytes = bytes loaded.
+ * *ms is the probe kernel's duration (HIP events on the projector's stream). */
+typedef enum ctlproj_probe_mode { CTLPROJ_PROBE_GATHER = 0, CTLPROJ_PROBE_STREAM = 1 } ctlproj_probe_mode;
+ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_volume* vol, int mode,
+                                         float* device_out, size_t out_count, float* ms, double* bytes);
+
 /* ---- failure injection (tests of the error paths) --------------------------------------- */
 
 /* The next `count` device operations of the given context fail with CTLPROJ_ERR_INJECTED. */

@@ -36,6 +36,8 @@ class Settings(C.Structure):
         ("tile_h", C.c_uint32),
         ("unroll", C.c_uint32),
         ("blocks_per_cu", C.c_uint32),
+        ("lds_chunk", C.c_int32),
+        ("lds_texels", C.c_uint32),
     ]
 
 

@@ -74,21 +74,35 @@ ctlproj_status validateSettings(const ctlproj_settings& s)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unroll must be 0, 1, 2 or 4");
     if (s.blocks_per_cu > 8)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "blocks_per_cu must be 0..8");
+    if (s.lds_chunk != 0 && s.lds_chunk != -1 && (s.lds_chunk < 16 || s.lds_chunk > 1024))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "lds_chunk must be 0, -1 or 16..1024");
+    if (s.lds_texels != 0 && (s.lds_texels < 1024 || s.lds_texels > ctlk::kMaxLdsTexels))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "lds_texels must be 0 or 1024.." + std::to_string(ctlk::kMaxLdsTexels));
     return CTLPROJ_OK;
+}
+
+bool stagedWorkload(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly)
+{
+    return st == ctlk::Storage::F32 && ly == ctlk::Layout::Interleaved && s.interpolate &&
+           s.rays_per_pixel[0] == 1 && s.rays_per_pixel[1] == 1 && s.lds_chunk >= 0;
 }
 
 ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly,
                            uint32_t channels, uint32_t rows)
 {
-    // explicit settings win; otherwise the autotuner's choice for this workload class, if any
-    ctlk::LaunchShape sh;
-    ctltune::lookup(ctltune::key(st, ly, s.interpolate ? 1 : 0, channels, rows), &sh);
+    // explicit settings win; otherwise the autotuner's choice for this workload class, if any,
+    // else the default shape of the kernel that applies
+    const bool staged = stagedWorkload(s, st, ly);
+    ctlk::LaunchShape sh = ctlk::defaultShape(staged);
+    ctltune::lookup(ctltune::key(st, ly, s.interpolate ? 1 : 0, channels, rows, staged), &sh);
     if (s.tile_w && s.tile_h) {
         sh.wave_w = s.tile_w;
         sh.wave_h = s.tile_h;
     }
     if (s.unroll) sh.unroll = s.unroll;
     if (s.blocks_per_cu) sh.blocks_per_cu = s.blocks_per_cu;
+    if (s.lds_chunk) sh.chunk = s.lds_chunk < 0 ? 0u : uint32_t(s.lds_chunk);
+    if (s.lds_texels) sh.lds_texels = s.lds_texels & ~127u;
     return sh;
 }
 
@@ -714,9 +728,10 @@ ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_vo
         return r == hipSuccess ? ms : -1.0f;
     };
     ctltune::Choice best;
+    const bool staged = stagedWorkload(p->s, ev.view.storage, ev.view.layout);
     const bool ok = ctltune::tune(ctltune::key(ev.view.storage, ev.view.layout, p->s.interpolate ? 1 : 0,
-                                               p->C, p->R),
-                                  run, &best);
+                                               p->C, p->R, staged),
+                                  run, &best, staged);
     (void)hipFree(d_g);
     (void)hipFree(d_o);
     if (!ok) return fail(CTLPROJ_ERR_DEVICE, "autotuning failed: no candidate ran");

@@ -14,14 +14,29 @@ std::mutex g_mu;
 std::map<std::string, Choice> g_table;
 } // namespace
 
-std::vector<ctlk::LaunchShape> candidates()
+std::vector<ctlk::LaunchShape> candidates(bool staged)
 {
     std::vector<ctlk::LaunchShape> out;
+    if (staged) {
+        const uint32_t tiles[2][2] = { { 16, 4 }, { 8, 8 } };
+        for (const auto& t : tiles)
+            for (uint32_t chunk : { 64u, 48u })
+                for (uint32_t texels : { 2560u, 2048u, 3200u }) {
+                    ctlk::LaunchShape s = ctlk::defaultShape(true);
+                    s.wave_w = t[0];
+                    s.wave_h = t[1];
+                    s.chunk = chunk;
+                    s.lds_texels = texels;
+                    out.push_back(s);
+                }
+        out.push_back(ctlk::defaultShape(false));
+        return out;
+    }
     const uint32_t tiles[3][2] = { { 8, 8 }, { 16, 4 }, { 4, 16 } };
     for (const auto& t : tiles)
         for (uint32_t u : { 2u, 4u })
             for (uint32_t occ : { 0u, 4u }) {
-                ctlk::LaunchShape s;
+                ctlk::LaunchShape s = ctlk::defaultShape(false);
                 s.wave_w = t[0];
                 s.wave_h = t[1];
                 s.block_waves_x = 2;
@@ -34,11 +49,11 @@ std::vector<ctlk::LaunchShape> candidates()
 }
 
 bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
-          Choice* best)
+          Choice* best, bool staged)
 {
     Choice b;
     b.ms = -1.0f;
-    for (const auto& s : candidates()) {
+    for (const auto& s : candidates(staged)) {
         run(s); // first launch of a shape warms its code object and caches
         const float ms = run(s);
         if (ms < 0.0f) continue;
@@ -63,12 +78,13 @@ bool lookup(const std::string& key, ctlk::LaunchShape* shape)
     return true;
 }
 
-std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows)
+std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows,
+                bool staged)
 {
     // detector size class: number of 256-pixel steps of the larger side
     const uint32_t cls = ((channels > rows ? channels : rows) + 255) / 256;
     std::ostringstream os;
-    os << "s" << int(st) << "_l" << int(ly) << "_i" << interpolate << "_d" << cls;
+    os << "s" << int(st) << "_l" << int(ly) << "_i" << interpolate << "_d" << cls << (staged ? "_lds" : "");
     return os.str();
 }
 
@@ -83,7 +99,8 @@ std::string exportTable()
         first = false;
         const auto& s = kv.second.shape;
         os << "\"" << kv.first << "\":{\"tile_w\":" << s.wave_w << ",\"tile_h\":" << s.wave_h
-           << ",\"unroll\":" << s.unroll << ",\"blocks_per_cu\":" << s.blocks_per_cu << ",\"ms\":" << kv.second.ms << "}";
+           << ",\"unroll\":" << s.unroll << ",\"blocks_per_cu\":" << s.blocks_per_cu << ",\"lds_chunk\":" << s.chunk
+           << ",\"lds_texels\":" << s.lds_texels << ",\"ms\":" << kv.second.ms << "}";
     }
     os << "}";
     return os.str();

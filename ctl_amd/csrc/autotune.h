@@ -14,20 +14,24 @@ struct Choice {
     float ms = 0.0f;
 };
 
-// Candidate shapes: wavefront tiles 8x8, 16x4, 4x16 (pixels along channels x rows), unroll 2, 4
-// and a cap of 0 (none) or 4 resident workgroups per CU.
-std::vector<ctlk::LaunchShape> candidates();
+// Candidate shapes. Gathers from HBM: wavefront tiles 8x8, 16x4, 4x16 (pixels along channels x
+// rows), unroll 2, 4 and a cap of 0 (none) or 4 resident workgroups per CU. LDS-staged workloads
+// (`staged`): tiles 8x8, 16x4 with 48- or 64-sample chunks in LDS boxes of 2048, 2560 or 3200
+// texels (unroll 2), and the best unstaged shape.
+std::vector<ctlk::LaunchShape> candidates(bool staged);
 
 // Times every candidate with `run` (returns milliseconds, or a negative value on failure), keeps
 // the fastest and records it in the process-wide table under `key`.
 bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
-          Choice* best);
+          Choice* best, bool staged = false);
 
 // Shape recorded for `key` by an earlier tune(); false if none.
 bool lookup(const std::string& key, ctlk::LaunchShape* shape);
 
-// Key of a workload class: storage, layout, interpolation and detector size class.
-std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows);
+// Key of a workload class: storage, layout, interpolation, detector size class and whether the
+// LDS-staged kernel applies.
+std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows,
+                bool staged);
 
 // The whole table as JSON text.
 std::string exportTable();

@@ -43,6 +43,7 @@ struct ctlproj_batch {
     ctlk::Layout ly = ctlk::Layout::Interleaved;
     int interp = 1;
     ctlk::LaunchShape shape;
+    bool staged = false; // some item can use the LDS-staged kernel
     uint32_t grid = 0;
     std::vector<ctlk::BatchItem> items;
     ctlk::BatchItem* d_items = nullptr;
@@ -84,6 +85,7 @@ ctlproj_status buildItems(ctlproj_batch* b, const ctlk::LaunchShape& sh)
         it.out = in->out;
         it.rp = rayParams(*in);
         it.kc = ctlk::makeCastConst(&in->ev.view, in->ev.geom, it.rp, sh);
+        if (!stagedWorkload(in->s, in->ev.view.storage, in->ev.view.layout)) it.kc.chunk = 0; // lds_chunk -1
         it.tile_begin = begin;
         begin += (unsigned long long)it.kc.tiles_per_vm * in->V * in->M;
         b->items.push_back(it);
@@ -113,18 +115,25 @@ ctlproj_status enqueueSetups(ctlproj_batch* b)
     return CTLPROJ_OK;
 }
 
+// LDS box of the batch launch: 0 selects the unstaged kernel.
+uint32_t ldsTexels(const ctlproj_batch* b, const ctlk::LaunchShape& sh)
+{
+    return b->staged && sh.chunk ? sh.lds_texels : 0u;
+}
+
 // One timed launch over every `stride`-th tile (autotuner).
 float timedSample(ctlproj_batch* b, const ctlk::LaunchShape& sh, uint32_t stride)
 {
     if (buildItems(b, sh) != CTLPROJ_OK) return -1.0f;
     uint32_t grid = 0;
-    if (ctlk::batchGrid(b->st, b->ly, b->interp, sh.unroll, sh.blocks_per_cu, &grid) != hipSuccess) return -1.0f;
+    if (ctlk::batchGrid(b->st, b->ly, b->interp, sh.unroll, sh.blocks_per_cu, ldsTexels(b, sh), &grid) != hipSuccess)
+        return -1.0f;
     const unsigned long long n = (b->total_tiles + stride - 1) / stride;
     hipError_t e = ctlk::launchQueueInit(b->d_q, n, b->stream);
     if (e == hipSuccess) e = hipEventRecord(b->e0, b->stream);
     if (e == hipSuccess)
         e = ctlk::launchBatch(b->d_items, uint32_t(b->items.size()), b->st, b->ly, b->interp, sh.unroll,
-                              sh.blocks_per_cu, grid, b->d_q, stride, b->stream);
+                              sh.blocks_per_cu, ldsTexels(b, sh), grid, b->d_q, stride, b->stream);
     if (e == hipSuccess) e = hipEventRecord(b->e1, b->stream);
     if (e == hipSuccess) e = hipEventSynchronize(b->e1);
     float ms = -1.0f;
@@ -243,14 +252,17 @@ ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune)
 
     // launch shape: tuned on a sample of this batch's own tiles, or the table / default
     const ctlk::Storage s0 = b->st;
-    const std::string key = ctltune::key(s0, b->ly, b->interp, first.C, first.R) + "_batch";
-    ctlk::LaunchShape sh;
+    b->staged = false;
+    for (const auto& in : b->insts) b->staged = b->staged || stagedWorkload(in->s, s0, b->ly);
+    const std::string key = ctltune::key(s0, b->ly, b->interp, first.C, first.R, b->staged) + "_batch";
+    ctlk::LaunchShape sh = ctlk::defaultShape(b->staged);
     bool have = false;
     if (autotune) {
         if ((st = buildItems(b, sh)) != CTLPROJ_OK) return st;
         const uint32_t stride = uint32_t(std::max<unsigned long long>(1, b->total_tiles / 65536));
         ctltune::Choice best;
-        have = ctltune::tune(key, [&](const ctlk::LaunchShape& c) { return timedSample(b, c, stride); }, &best);
+        have = ctltune::tune(key, [&](const ctlk::LaunchShape& c) { return timedSample(b, c, stride); }, &best,
+                             b->staged);
         if (have) {
             sh = best.shape;
             b->tuned_ms = best.ms;
@@ -259,7 +271,8 @@ ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune)
     if (!have) ctltune::lookup(key, &sh);
     b->shape = sh;
     if ((st = buildItems(b, sh)) != CTLPROJ_OK) return st;
-    if ((e = ctlk::batchGrid(b->st, b->ly, b->interp, sh.unroll, sh.blocks_per_cu, &b->grid)) != hipSuccess)
+    if ((e = ctlk::batchGrid(b->st, b->ly, b->interp, sh.unroll, sh.blocks_per_cu, ldsTexels(b, sh), &b->grid)) !=
+        hipSuccess)
         return hipFail(e, "occupancy query");
 
     // capture: per-view setups, queue reset, persistent kernel (kernels only, no memset nodes)
@@ -270,7 +283,7 @@ ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune)
     if (st == CTLPROJ_OK) ec = ctlk::launchQueueInit(b->d_q, b->total_tiles, b->stream);
     if (st == CTLPROJ_OK && ec == hipSuccess)
         ec = ctlk::launchBatch(b->d_items, uint32_t(b->items.size()), b->st, b->ly, b->interp, sh.unroll,
-                               sh.blocks_per_cu, b->grid, b->d_q, 1, b->stream);
+                               sh.blocks_per_cu, ldsTexels(b, sh), b->grid, b->d_q, 1, b->stream);
     e = hipStreamEndCapture(b->stream, &b->graph);
     if (st != CTLPROJ_OK) return st;
     if (ec != hipSuccess) return hipFail(ec, "batch capture");

@@ -61,12 +61,39 @@ struct LaunchShape {
     uint32_t block_waves_x = 2, block_waves_y = 2; // waves per workgroup
     uint32_t unroll = 4;
     uint32_t blocks_per_cu = 0; // 0: as many as registers allow; else capped with reserved LDS
+    // LDS staging (FP32 interleaved volumes, trilinear, one ray per pixel): samples per chunk
+    // (0 = unstaged) and LDS box capacity in texels (a multiple of 128)
+    uint32_t chunk = 64;
+    uint32_t lds_texels = 2560; // 20 KiB: seven workgroups per CU
 };
 
-// Dynamic LDS that caps the resident workgroups per CU at `blocks` (160 KiB of LDS per CU).
+// Default launch shape: 8x8 wavefront tiles unrolled 4x for the gathers from HBM; 16x4 tiles
+// unrolled 2x with 64-sample chunks in a 2560-texel LDS box for the staged kernel (the fastest
+// shapes measured on the 256^3 / 1024x768 workload).
+inline LaunchShape defaultShape(bool staged)
+{
+    LaunchShape s;
+    if (staged) {
+        s.wave_w = 16;
+        s.wave_h = 4;
+        s.unroll = 2;
+    } else {
+        s.chunk = 0;
+    }
+    return s;
+}
+
+// LDS of a CU and of one workgroup on gfx950; kernels keep at most kStaticLds bytes of static LDS
+// beside the dynamic allocation.
+constexpr uint32_t kLdsBytes = 160u * 1024u;
+constexpr uint32_t kStaticLds = 1024u;
+// Largest LDS box of the staged kernel, in 8-byte texels (a multiple of 128).
+constexpr uint32_t kMaxLdsTexels = ((kLdsBytes - kStaticLds) / 8u) & ~127u;
+
+// Dynamic LDS that caps the resident workgroups per CU at `blocks`.
 inline uint32_t ldsForBlocks(uint32_t blocks)
 {
-    return blocks ? ((160u * 1024u / blocks) / 256u) * 256u : 0u;
+    return blocks ? (((kLdsBytes - kStaticLds) / blocks) / 256u) * 256u : 0u;
 }
 
 struct RayParams {
@@ -74,6 +101,13 @@ struct RayParams {
     uint32_t rpp_x, rpp_y;
     int interpolate;
 };
+
+// True if a projection with this storage, layout and mode can use the LDS-staged kernel.
+inline bool stageable(Storage st, Layout ly, const RayParams& rp)
+{
+    return st == Storage::F32 && ly == Layout::Interleaved && rp.interpolate && rp.rpp_x == 1 &&
+           rp.rpp_y == 1;
+}
 
 struct Pitch {
     uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) texel rows
@@ -89,6 +123,8 @@ struct CastConst {
     float nhi[3]; // n + 1 per axis
     float blo;    // 0 (trilinear) or 1 (nearest)
     Pitch pitch;
+    uint32_t chunk;      // LDS-staged kernel: samples per ray and chunk (0: unstaged kernel)
+    uint32_t lds_texels; // LDS-staged kernel: capacity of the LDS box in 8-byte texels
 };
 
 CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
@@ -113,12 +149,14 @@ struct BatchQueue {
 // Resets the queue to [0, total) split over the XCDs (a kernel, so it can sit in a graph).
 hipError_t launchQueueInit(BatchQueue* d_q, unsigned long long total, hipStream_t s);
 // Persistent kernel over `n_items` projections; all items share storage, layout and mode.
+// lds_texels > 0 with FP32 interleaved trilinear items selects the LDS-staged kernel with that box
+// (items with kc.chunk == 0 gather from HBM inside it); 0 selects the unstaged kernel.
 hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, Layout ly,
-                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t grid,
-                       BatchQueue* d_q, uint32_t stride, hipStream_t s);
-// Workgroups of the batch kernel that fit on the device at once.
+                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t lds_texels,
+                       uint32_t grid, BatchQueue* d_q, uint32_t stride, hipStream_t s);
+// Workgroups of the batch kernel (as selected by launchBatch) that fit on the device at once.
 hipError_t batchGrid(Storage st, Layout ly, int interpolate, uint32_t unroll, uint32_t blocks_per_cu,
-                     uint32_t* grid);
+                     uint32_t lds_texels, uint32_t* grid);
 
 // FP64 setup: one thread per (view, module).
 hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,

@@ -443,6 +443,210 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
                                      threadIdx.x >> 6);
 }
 
+// ---------------------------------------------------------------------------------------------
+// LDS-staged ray casting (FP32 interleaved volume, trilinear, one ray per pixel)
+//
+// The 256 rays of a workgroup walk their sample indices together in chunks of kc.chunk samples.
+// For each chunk the workgroup takes the exact box of texels that its rays' taps touch (sample
+// positions are monotone in i, so the chunk's end samples bound it), copies that box from HBM into
+// LDS with direct buffer-to-LDS loads, and samples from LDS. A gather from LDS costs a fraction
+// of a gather through the vector L1, whose scattered cache lines bound the unstaged kernel. A
+// chunk whose box does not fit the LDS budget samples from HBM as the unstaged kernel does.
+// Samples, their order within a ray and the FP32 arithmetic of each sample are the same as in
+// castTile.
+
+// Wave-wide min / max: DPP reductions within each 16-lane row, then the four row results. All
+// 64 lanes must be active.
+template <bool MAX>
+__device__ __forceinline__ int waveReduce(int v)
+{
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false)); // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false)); // row_ror:8
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Trilinear sample from the LDS box (z-pair texels) at byte offset `ofs`, with rows px8 bytes
+// apart; the same lerps in the same order as Sampler<F32, Interleaved>.
+__device__ __forceinline__ float triLds(const float2* __restrict__ box, uint32_t ofs, uint32_t px8,
+                                        float wx, float wy, float wz)
+{
+    const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(box) + ofs);
+    const float2* q = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(p) + px8);
+    const float2 a0 = p[0], a1 = p[1];
+    const float2 b0 = q[0], b1 = q[1];
+    const float c00 = lerp(a0.x, a1.x, wx), c01 = lerp(a0.y, a1.y, wx);
+    const float c10 = lerp(b0.x, b1.x, wx), c11 = lerp(b0.y, b1.y, wx);
+    return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+}
+
+// Samples [wlo, whi] of a chunk for a lane whose own range is [a, e] (`live` if non-empty), with
+// [wa, we] the range every live lane of the wave shares: head and tail under the exec mask, the
+// shared body unmasked and unrolled.
+template <int UNROLL, typename F>
+__device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, int a, int e, bool live,
+                                            float* part, F&& f)
+{
+    const uint32_t span = uint32_t(e - a);
+    for (int i = wlo; i < wa; ++i)
+        if (live && uint32_t(i - a) <= span) part[0] += f(float(i));
+    if (live) {
+        int i = wa;
+        float fi = float(wa);
+        for (; i + UNROLL - 1 <= we; i += UNROLL, fi += float(UNROLL)) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) part[u] += f(fi + float(u));
+        }
+        for (; i <= we; ++i, fi += 1.0f) part[0] += f(fi);
+    }
+    for (int i = we + 1; i <= whi; ++i)
+        if (live && uint32_t(i - a) <= span) part[0] += f(float(i));
+}
+
+template <int UNROLL>
+__device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
+                                               const RayParams& rp, const CastConst& kc,
+                                               float* __restrict__ out, uint32_t b, uint32_t wave,
+                                               float2* __restrict__ box, int (*red)[4][8])
+{
+    const uint32_t vm = b / kc.tiles_per_vm;
+    const uint32_t t = b - vm * kc.tiles_per_vm;
+    const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
+    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
+    const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
+    const bool pixel = x < rp.channels && y < rp.rows;
+
+    const RayGeom g = geom[vm];
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+    float acc = 0.0f;
+    if (g.valid) { // uniform over the workgroup: one (view, module) per workgroup
+        // one ray per pixel: the sub-ray coordinate x - 1/2 + 1/2 is x exactly
+        Ray ray = setupRay(g, float(x), float(y), kc.inc, kc.vs, kc.nhi, kc.blo);
+        if (!pixel) ray.hit = false;
+        tighten(ray, g.c, kc.blo, kc.nhi);
+        const bool act = pixel && ray.i0 <= ray.i1;
+        // sample range of the workgroup
+        {
+            const int lo = waveReduce<false>(act ? ray.i0 : INT_MAX);
+            const int hi = waveReduce<true>(act ? ray.i1 : INT_MIN);
+            if (lane == 0) { red[1][wave][0] = lo; red[1][wave][1] = hi; }
+        }
+        __syncthreads();
+        const int blo = min(min(red[1][0][0], red[1][1][0]), min(red[1][2][0], red[1][3][0]));
+        const int bhi = max(max(red[1][0][1], red[1][1][1]), max(red[1][2][1], red[1][3][1]));
+        const int K = int(kc.chunk);
+        float part[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
+        int parity = 0;
+        for (int c0 = blo; c0 <= bhi; c0 += K, parity ^= 1) {
+            const int c1 = min(c0 + K - 1, bhi);
+            const int a = act ? max(c0, ray.i0) : INT_MAX;
+            const int e = act ? min(c1, ray.i1) : INT_MIN;
+            const bool live = a <= e;
+            // texel box of this lane's taps in the chunk: [lo, hi] per axis (z-pair texels carry
+            // z+1, so z needs no extra slice)
+            int bl[3], bh[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float qa = fmaf(float(a), ray.s[k], g.c[k]), qe = fmaf(float(e), ray.s[k], g.c[k]);
+                bl[k] = live ? int(fminf(qa, qe)) : INT_MAX;
+                bh[k] = live ? int(fmaxf(qa, qe)) + (k < 2 ? 1 : 0) : INT_MIN;
+            }
+            const int wlo = waveReduce<false>(live ? a : INT_MAX), whi = waveReduce<true>(live ? e : INT_MIN);
+            int wa = waveReduce<true>(live ? a : INT_MIN), we = waveReduce<false>(live ? e : INT_MAX);
+            if (wlo > whi || wa > we) { wa = whi + 1; we = whi; } // no shared range: all masked
+            {
+                int r[6];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    r[k] = waveReduce<false>(bl[k]);
+                    r[3 + k] = waveReduce<true>(bh[k]);
+                }
+                if (lane == 0)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) red[parity][wave][k] = r[k];
+            }
+            __syncthreads(); // the box of the previous chunk is no longer read; red[parity] is complete
+            int o[3], nb[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                o[k] = min(min(red[parity][0][k], red[parity][1][k]), min(red[parity][2][k], red[parity][3][k]));
+                const int h = max(max(red[parity][0][3 + k], red[parity][1][3 + k]),
+                                  max(red[parity][2][3 + k], red[parity][3][3 + k]));
+                nb[k] = o[k] == INT_MAX ? 0 : h - o[k] + 1; // positions lie in [0, n+1]: no overflow
+            }
+            if (nb[0] == 0) continue; // no ray of the workgroup samples in this chunk
+            // LDS pitches: even rows (a 16-byte load never straddles two rows), not a multiple of
+            // 16 texels (rows would share LDS banks)
+            int px = (nb[0] + 1) & ~1;
+            if ((px & 15) == 0) px += 2;
+            const bool small = nb[0] <= 256 && nb[1] <= 256 && nb[2] <= 256;
+            const int pz = px * nb[1];
+            const int total = small ? pz * nb[2] : INT_MAX;
+            const int n_ins = small ? (total + 127) >> 7 : INT_MAX; // 16-byte loads of 64 lanes = 128 texels
+            if (small && n_ins * 128 <= int(kc.lds_texels)) {
+                const float rpx = 1.0f / float(px), rby = 1.0f / float(nb[1]);
+                for (int k = int(wave); k < n_ins; k += 4) {
+                    const int tt = k * 128 + 2 * int(lane);
+                    int row = int(float(tt) * rpx);
+                    int cx = tt - row * px;
+                    if (cx < 0) { --row; cx += px; } else if (cx >= px) { ++row; cx -= px; }
+                    int cz = int(float(row) * rby);
+                    int cy = row - cz * nb[1];
+                    if (cy < 0) { --cz; cy += nb[1]; } else if (cy >= nb[1]) { ++cz; cy -= nb[1]; }
+                    // texels past the box land in padding of the LDS box; past the volume they read 0
+                    const uint32_t src = (uint32_t(o[0] + cx) + uint32_t(o[1] + cy) * kc.pitch.ty +
+                                          uint32_t(o[2] + cz) * kc.pitch.tz) << 3;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(box + k * 128), 16, src, 0, 0, 0);
+                }
+                __syncthreads(); // box loaded (the compiler waits for the loads before the barrier)
+                // byte offset of texel (ix, iy, iz) in the box: pitches in bytes, so that the address
+                // is one shift-add and two 24-bit multiply-adds (mod 2^32; the result is in range)
+                const uint32_t px8 = uint32_t(px) * 8u, pz8 = uint32_t(pz) * 8u;
+                const uint32_t base8 = (uint32_t(o[0]) + uint32_t(o[1]) * uint32_t(px) + uint32_t(o[2]) * uint32_t(pz)) * 8u;
+                sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, [&](float fi) {
+                    const float qx = fmaf(fi, ray.s[0], g.c[0]);
+                    const float qy = fmaf(fi, ray.s[1], g.c[1]);
+                    const float qz = fmaf(fi, ray.s[2], g.c[2]);
+                    const uint32_t ofs = __umul24(uint32_t(int(qz)), pz8) +
+                                         (__umul24(uint32_t(int(qy)), px8) + ((uint32_t(int(qx)) << 3) - base8));
+                    return triLds(box, ofs, px8, __builtin_amdgcn_fractf(qx), __builtin_amdgcn_fractf(qy),
+                                  __builtin_amdgcn_fractf(qz));
+                });
+            } else {
+                sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, [&](float fi) {
+                    return sampleAt<Storage::F32, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
+                });
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) acc += part[u];
+    }
+    if (pixel)
+        out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+}
+
+template <int UNROLL>
+__global__ __launch_bounds__(kBlock) void staged_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
+                                                        RayParams rp, CastConst kc, float* __restrict__ out)
+{
+    extern __shared__ __attribute__((aligned(16))) float2 s_box[];
+    __shared__ int s_red[2][4][8];
+    static_assert(sizeof(s_red) <= kStaticLds, "static LDS beyond the reserve");
+    castTileStaged<UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
+                           s_box, s_red);
+}
+
 // Persistent batch kernel: a fixed grid of workgroups drains a queue of tiles that spans many
 // projections (volumes, geometries and detector sizes of their own). A tile is a workgroup's
 // 2x2 wavefronts of neighbouring pixels, so the four waves on a CU gather from the same patch
@@ -450,6 +654,43 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
 // from its own XCD's range first (neighbouring tiles of one view then share that XCD's L2) and
 // then from the others. `stride` > 1 visits every stride-th tile (used by the autotuner to time a
 // sample of the work).
+// Takes the next tile for workgroup-home XCD `home` (own XCD's range first, then the others):
+// sets *t to the tile (~0 once every range is drained) and *k to its item. Called by all threads.
+__device__ __forceinline__ void grabTile(const BatchItem* __restrict__ items, uint32_t n_items,
+                                         BatchQueue* __restrict__ q, uint32_t stride, uint32_t home,
+                                         unsigned long long* s_tile, uint32_t* s_item,
+                                         unsigned long long* t, uint32_t* k)
+{
+    if (threadIdx.x == 0) {
+        unsigned long long got = ~0ull;
+        for (uint32_t j = 0; j < kNumXcd && got == ~0ull; ++j) {
+            const uint32_t x = (home + j) % kNumXcd;
+            if (__hip_atomic_load(&q->next[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q->end[x]) continue;
+            const unsigned long long tt = atomicAdd(&q->next[x], 1ull);
+            if (tt < q->end[x]) got = tt;
+        }
+        uint32_t lo = 0;
+        if (got != ~0ull) {
+            const unsigned long long tt = got * stride;
+            uint32_t hi = n_items; // last item with tile_begin <= tt
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (items[mid].tile_begin <= tt) lo = mid;
+                else hi = mid;
+            }
+        }
+        *s_tile = got;
+        *s_item = lo;
+    }
+    __syncthreads();
+    // wave-uniform copies: the item's fields are then scalar loads
+    const uint32_t tlo = __builtin_amdgcn_readfirstlane(uint32_t(*s_tile));
+    const uint32_t thi = __builtin_amdgcn_readfirstlane(uint32_t(*s_tile >> 32));
+    *k = __builtin_amdgcn_readfirstlane(*s_item);
+    __syncthreads();
+    *t = (unsigned long long)thi << 32 | tlo;
+}
+
 template <Storage ST, Layout LY, bool INTERP, int UNROLL>
 __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem* __restrict__ items, uint32_t n_items,
                                                        BatchQueue* __restrict__ q, uint32_t stride)
@@ -458,38 +699,42 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem* __restri
     __shared__ uint32_t s_item;
     const uint32_t home = blockIdx.x % kNumXcd;
     for (;;) {
-        if (threadIdx.x == 0) {
-            unsigned long long got = ~0ull;
-            for (uint32_t k = 0; k < kNumXcd && got == ~0ull; ++k) {
-                const uint32_t x = (home + k) % kNumXcd;
-                if (__hip_atomic_load(&q->next[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= q->end[x]) continue;
-                const unsigned long long t = atomicAdd(&q->next[x], 1ull);
-                if (t < q->end[x]) got = t;
-            }
-            uint32_t lo = 0;
-            if (got != ~0ull) {
-                const unsigned long long tt = got * stride;
-                uint32_t hi = n_items; // last item with tile_begin <= tt
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) / 2;
-                    if (items[mid].tile_begin <= tt) lo = mid;
-                    else hi = mid;
-                }
-            }
-            s_tile = got;
-            s_item = lo;
-        }
-        __syncthreads();
-        // wave-uniform copies: the item's fields are then scalar loads
-        const uint32_t tlo = __builtin_amdgcn_readfirstlane(uint32_t(s_tile));
-        const uint32_t thi = __builtin_amdgcn_readfirstlane(uint32_t(s_tile >> 32));
-        const uint32_t k = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
-        const unsigned long long t = (unsigned long long)thi << 32 | tlo;
+        unsigned long long t;
+        uint32_t k;
+        grabTile(items, n_items, q, stride, home, &s_tile, &s_item, &t, &k);
         if (t == ~0ull) break; // every workgroup exits once all ranges are drained
         const BatchItem& it = items[k];
         castTile<ST, LY, INTERP, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out,
                                          uint32_t(t * stride - it.tile_begin), threadIdx.x >> 6);
+    }
+}
+
+// The same queue over FP32 interleaved trilinear items: tiles of items with kc.chunk > 0 run the
+// LDS-staged caster (kc.lds_texels must not exceed the launch's dynamic LDS), the others gather
+// from HBM.
+template <int UNROLL>
+__global__ __launch_bounds__(kBlock) void batch_staged_kernel(const BatchItem* __restrict__ items,
+                                                              uint32_t n_items, BatchQueue* __restrict__ q,
+                                                              uint32_t stride)
+{
+    extern __shared__ __attribute__((aligned(16))) float2 s_box[];
+    __shared__ int s_red[2][4][8];
+    __shared__ unsigned long long s_tile;
+    __shared__ uint32_t s_item;
+    static_assert(sizeof(s_red) + 16 <= kStaticLds, "static LDS beyond the reserve");
+    const uint32_t home = blockIdx.x % kNumXcd;
+    for (;;) {
+        unsigned long long t;
+        uint32_t k;
+        grabTile(items, n_items, q, stride, home, &s_tile, &s_item, &t, &k);
+        if (t == ~0ull) break;
+        const BatchItem& it = items[k];
+        const uint32_t tile = uint32_t(t * stride - it.tile_begin);
+        if (it.kc.chunk)
+            castTileStaged<UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out, tile, threadIdx.x >> 6, s_box, s_red);
+        else
+            castTile<Storage::F32, Layout::Interleaved, true, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out,
+                                                                      tile, threadIdx.x >> 6);
     }
 }
 
@@ -789,6 +1034,8 @@ CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayPa
         kc.nhi[k] = float(vg.n[k]) + 1.0f;
     }
     kc.blo = rp.interpolate ? 0.0f : 1.0f;
+    kc.chunk = (vol && stageable(vol->storage, vol->layout, rp)) ? sh.chunk : 0u;
+    kc.lds_texels = sh.lds_texels;
     if (vol) {
         const uint32_t eb = uint32_t(texelBytes(vol->storage, vol->layout));
         kc.pitch.ty = vol->pitch_y;
@@ -882,6 +1129,17 @@ hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayG
     if (blocks == 0) return hipSuccess;
     if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
     const uint32_t lds = ldsForBlocks(shape.blocks_per_cu);
+    if (kc.chunk) {
+        if (kc.lds_texels < 1024 || kc.lds_texels > kMaxLdsTexels || kc.lds_texels % 128) return hipErrorInvalidValue;
+        const uint32_t box = kc.lds_texels * 8;
+        const uint32_t dyn = box > lds ? box : lds;
+        switch (shape.unroll) {
+        case 1: staged_kernel<1><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
+        case 4: staged_kernel<4><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
+        default: staged_kernel<2><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
+        }
+        return hipGetLastError();
+    }
     switch (vol.storage) {
     case Storage::F16: return launchCastI<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
     case Storage::BF16: return launchCastI<Storage::BF16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
@@ -966,6 +1224,15 @@ const void* batchKernelS(Layout ly, int interp, uint32_t unroll)
     return interp ? batchKernelU<ST, Layout::Plain, true>(unroll) : batchKernelU<ST, Layout::Plain, false>(unroll);
 }
 
+const void* batchStagedKernel(uint32_t unroll)
+{
+    switch (unroll) {
+    case 1: return reinterpret_cast<const void*>(&batch_staged_kernel<1>);
+    case 4: return reinterpret_cast<const void*>(&batch_staged_kernel<4>);
+    default: return reinterpret_cast<const void*>(&batch_staged_kernel<2>);
+    }
+}
+
 const void* batchKernel(Storage st, Layout ly, int interp, uint32_t unroll)
 {
     switch (st) {
@@ -974,30 +1241,49 @@ const void* batchKernel(Storage st, Layout ly, int interp, uint32_t unroll)
     default: return batchKernelS<Storage::F32>(ly, interp, unroll);
     }
 }
+
+// Dynamic LDS of a batch launch: the staged kernel's box, or the occupancy-capping reservation.
+uint32_t batchLds(uint32_t lds_texels, uint32_t blocks_per_cu)
+{
+    const uint32_t cap = ldsForBlocks(blocks_per_cu), box = lds_texels * 8u;
+    return box > cap ? box : cap;
+}
+
+bool batchStaged(Storage st, Layout ly, int interp, uint32_t lds_texels)
+{
+    return lds_texels && st == Storage::F32 && ly == Layout::Interleaved && interp;
+}
+
 } // namespace
 
 hipError_t batchGrid(Storage st, Layout ly, int interpolate, uint32_t unroll, uint32_t blocks_per_cu,
-                     uint32_t* grid)
+                     uint32_t lds_texels, uint32_t* grid)
 {
+    if (lds_texels > kMaxLdsTexels || lds_texels % 128) return hipErrorInvalidValue;
+    const bool staged = batchStaged(st, ly, interpolate, lds_texels);
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batchKernel(st, ly, interpolate, unroll),
-                                                         kBlock, ldsForBlocks(blocks_per_cu));
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, staged ? batchStagedKernel(unroll) : batchKernel(st, ly, interpolate, unroll), kBlock,
+            staged ? batchLds(lds_texels, blocks_per_cu) : ldsForBlocks(blocks_per_cu));
     if (e != hipSuccess) return e;
     *grid = uint32_t(cus * (per_cu > 0 ? per_cu : 1));
     return hipSuccess;
 }
 
 hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, Layout ly,
-                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t grid,
-                       BatchQueue* d_q, uint32_t stride, hipStream_t s)
+                       int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t lds_texels,
+                       uint32_t grid, BatchQueue* d_q, uint32_t stride, hipStream_t s)
 {
     if (n_items == 0 || grid == 0) return hipSuccess;
-    const void* fn = batchKernel(st, ly, interpolate, unroll);
+    if (lds_texels > kMaxLdsTexels || lds_texels % 128) return hipErrorInvalidValue;
+    const bool staged = batchStaged(st, ly, interpolate, lds_texels);
+    const void* fn = staged ? batchStagedKernel(unroll) : batchKernel(st, ly, interpolate, unroll);
     void* args[] = { const_cast<BatchItem**>(&d_items), &n_items, &d_q, &stride };
-    return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args, ldsForBlocks(blocks_per_cu), s);
+    return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args,
+                           staged ? batchLds(lds_texels, blocks_per_cu) : ldsForBlocks(blocks_per_cu), s);
 }
 
 } // namespace ctlk

@@ -74,6 +74,8 @@ ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings
                                hipStream_t stream, EffectiveVolume& ev);
 
 ctlproj_status validateSettings(const ctlproj_settings& s);
+// True if projections with these settings and this volume storage use the LDS-staged kernel.
+bool stagedWorkload(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly);
 ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly,
                            uint32_t channels, uint32_t rows);
 

@@ -146,7 +146,6 @@ struct ctlproj_server {
     hipStream_t prep = nullptr, copy = nullptr;
     Slot slots[kSlots];
     uint32_t next_slot = 0;
-    ctlk::LaunchShape shape;
     std::map<GroupKey, uint32_t> grid_cache;
 
     std::mutex mu;
@@ -291,6 +290,12 @@ void failChunksLocked(ctlproj_server* S, std::vector<Chunk>& chunks, ctlproj_sta
 void launchLocked(ctlproj_server* S, Slot& slot)
 {
     const GroupKey key = groupOf(S->ready.front().r);
+    // FP32 interleaved trilinear groups run the LDS-staged kernel (its requests with more than one
+    // ray per pixel gather from HBM inside it); the others the unstaged kernel
+    const bool staged = std::get<0>(key) == int(ctlk::Storage::F32) &&
+                        std::get<1>(key) == int(ctlk::Layout::Interleaved) && std::get<2>(key);
+    const ctlk::LaunchShape shape = ctlk::defaultShape(staged);
+    const uint32_t lds_texels = staged ? shape.lds_texels : 0u;
     uint64_t pixels = 0;
     unsigned long long tiles = 0;
     uint32_t n = 0;
@@ -317,7 +322,7 @@ void launchLocked(ctlproj_server* S, Slot& slot)
             b.rp.rpp_x = r->s.rays_per_pixel[0];
             b.rp.rpp_y = r->s.rays_per_pixel[1];
             b.rp.interpolate = r->s.interpolate ? 1 : 0;
-            b.kc = ctlk::makeCastConst(&b.vol, r->vg, b.rp, S->shape);
+            b.kc = ctlk::makeCastConst(&b.vol, r->vg, b.rp, shape);
             b.tile_begin = tiles;
             tiles += (unsigned long long)b.kc.tiles_per_vm * it->nv * r->M;
             pixels += px;
@@ -330,7 +335,7 @@ void launchLocked(ctlproj_server* S, Slot& slot)
     hipError_t e = hipSuccess;
     if (grid == 0)
         e = ctlk::batchGrid(ctlk::Storage(std::get<0>(key)), ctlk::Layout(std::get<1>(key)), std::get<2>(key),
-                            S->shape.unroll, S->shape.blocks_per_cu, &grid);
+                            shape.unroll, shape.blocks_per_cu, lds_texels, &grid);
     if (e == hipSuccess)
         e = hipMemcpyAsync(slot.d_items, slot.h_items, n * sizeof(ctlk::BatchItem), hipMemcpyHostToDevice,
                            slot.stream);
@@ -343,7 +348,7 @@ void launchLocked(ctlproj_server* S, Slot& slot)
     if (e == hipSuccess) e = ctlk::launchQueueInit(slot.d_q, tiles, slot.stream);
     if (e == hipSuccess)
         e = ctlk::launchBatch(slot.d_items, n, ctlk::Storage(std::get<0>(key)), ctlk::Layout(std::get<1>(key)),
-                              std::get<2>(key), S->shape.unroll, S->shape.blocks_per_cu, grid, slot.d_q, 1,
+                              std::get<2>(key), shape.unroll, shape.blocks_per_cu, lds_texels, grid, slot.d_q, 1,
                               slot.stream);
     if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
     if (e != hipSuccess) {

@@ -318,6 +318,8 @@ class Settings:
     tile: tuple | None = None  # (wave_w, wave_h), 0 = automatic
     unroll: int = 0
     blocks_per_cu: int = 0  # cap on resident workgroups per CU (0 = automatic)
+    lds_chunk: int = 0  # LDS staging, samples per chunk (0 = automatic, -1 = off)
+    lds_texels: int = 0  # LDS box capacity in texels (0 = automatic)
 
     def _c(self) -> _lib.Settings:
         s = _lib.Settings()
@@ -329,6 +331,8 @@ class Settings:
             s.tile_w, s.tile_h = self.tile
         s.unroll = self.unroll
         s.blocks_per_cu = self.blocks_per_cu
+        s.lds_chunk = self.lds_chunk
+        s.lds_texels = self.lds_texels
         return s
 
 

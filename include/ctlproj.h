@@ -58,6 +58,9 @@ typedef struct ctlproj_settings {
     uint32_t tile_h;   /* pixels per wavefront tile along rows (0 = auto) */
     uint32_t unroll;   /* ray-step unroll of the sampling loop (0 = auto) */
     uint32_t blocks_per_cu; /* cap on resident workgroups per CU, 1..8 (0 = auto) */
+    int32_t lds_chunk;      /* LDS staging of FP32 trilinear projections: samples per chunk,
+                               16..1024 (0 = auto, -1 = off: gather from HBM through L1) */
+    uint32_t lds_texels;    /* LDS box capacity in 8-byte texels, 1024..20352 (0 = auto) */
 } ctlproj_settings;
 
 typedef struct ctlproj_context ctlproj_context;

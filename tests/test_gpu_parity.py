@@ -90,6 +90,37 @@ def test_unroll_and_tile_variants_agree(small, small_refs):
             assert e <= L2_F32, (tile, unroll, e)
 
 
+@pytest.mark.parametrize("chunk,texels", [(16, 0), (64, 0), (256, 0), (64, 1024), (1024, 20352)])
+def test_lds_staged_matches_unstaged_and_ref32(small, small_refs, chunk, texels):
+    """FP32 trilinear projections stage the taps of each chunk of samples in LDS. Against the
+    unstaged kernel with the same tile and unroll only the split of the partial sums at chunk
+    boundaries differs; 1024 texels forces the HBM fallback for the larger boxes."""
+    r32 = small_refs[0]
+    pm = small.pmats[::5]
+    plain, _, _ = run_gpu(small, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
+    got, _, _ = run_gpu(small, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8), unroll=4)
+    d = oracle.rel_l2(plain, got)
+    e, m = oracle.rel_l2(r32[::5], got), oracle.max_abs_rel(r32[::5], got)
+    print(f"lds chunk {chunk} texels {texels}: vs unstaged {d:.3e}, vs ref32 {e:.3e} / {m:.3e}")
+    assert d <= 1e-6
+    assert e <= L2_F32 and m <= MAR_F32
+
+
+def test_lds_staged_real_size_oblique_views():
+    """Full-size geometry: views at 0, 45, 137 and 225 degrees (boxes of axis-aligned and
+    diagonal ray bundles), staged against unstaged and ref32."""
+    w = W.real(views=360)
+    pick = [0, 45, 137, 225]
+    r32, _ = run_ref(w, views=pick)
+    plain, _, _ = run_gpu(w, pm=w.pmats[pick], lds_chunk=-1, tile=(16, 4), unroll=2)
+    got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 2
+    d = oracle.rel_l2(plain, got)
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"lds real views {pick}: vs unstaged {d:.3e}, vs ref32 {e:.3e} / {m:.3e}")
+    assert d <= 1e-6
+    assert e <= L2_F32 and m <= MAR_F32
+
+
 def test_nearest_mode(small):
     pm = small.pmats[::4]
     r32, _ = run_ref(small, pm=pm, interpolate=False)
@@ -194,6 +225,16 @@ def test_error_paths():
     assert ei.value.status == _lib.ERR_INJECTED
     out = p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm))).data
     assert np.isfinite(out).all() and out.max() > 0
+    # an LDS box beyond one workgroup's LDS (with the kernels' static reserve) is refused on the
+    # host, before any launch
+    for bad in (dict(lds_texels=20480), dict(lds_chunk=8), dict(blocks_per_cu=9)):
+        q = RayCasterProjector()
+        for k, v in bad.items():
+            setattr(q.settings(), k, v)
+        with pytest.raises(CtlProjError) as ei:
+            q.configure((w.pmats, w.channels, w.rows))
+            q.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm)))
+        assert ei.value.status == _lib.ERR_INVALID_ARGUMENT, bad
 
 
 def ctypes_ref():

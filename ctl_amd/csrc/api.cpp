@@ -83,8 +83,10 @@ ctlproj_status validateSettings(const ctlproj_settings& s)
 
 bool stagedWorkload(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly)
 {
-    return st == ctlk::Storage::F32 && ly == ctlk::Layout::Interleaved && s.interpolate &&
-           s.rays_per_pixel[0] == 1 && s.rays_per_pixel[1] == 1 && s.lds_chunk >= 0;
+    // FP16/BF16 sampling is bound by its arithmetic, not by the gathers, and staging only adds
+    // work: it stages on request (lds_chunk > 0), FP32 by default
+    return ly == ctlk::Layout::Interleaved && s.interpolate && s.rays_per_pixel[0] == 1 &&
+           s.rays_per_pixel[1] == 1 && (st == ctlk::Storage::F32 ? s.lds_chunk >= 0 : s.lds_chunk > 0);
 }
 
 ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly,

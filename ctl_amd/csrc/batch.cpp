@@ -43,7 +43,7 @@ struct ctlproj_batch {
     ctlk::Layout ly = ctlk::Layout::Interleaved;
     int interp = 1;
     ctlk::LaunchShape shape;
-    bool staged = false; // some item can use the LDS-staged kernel
+    bool staged = false; // every item can use the LDS-staged kernel
     uint32_t grid = 0;
     std::vector<ctlk::BatchItem> items;
     ctlk::BatchItem* d_items = nullptr;
@@ -85,7 +85,6 @@ ctlproj_status buildItems(ctlproj_batch* b, const ctlk::LaunchShape& sh)
         it.out = in->out;
         it.rp = rayParams(*in);
         it.kc = ctlk::makeCastConst(&in->ev.view, in->ev.geom, it.rp, sh);
-        if (!stagedWorkload(in->s, in->ev.view.storage, in->ev.view.layout)) it.kc.chunk = 0; // lds_chunk -1
         it.tile_begin = begin;
         begin += (unsigned long long)it.kc.tiles_per_vm * in->V * in->M;
         b->items.push_back(it);
@@ -252,8 +251,8 @@ ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune)
 
     // launch shape: tuned on a sample of this batch's own tiles, or the table / default
     const ctlk::Storage s0 = b->st;
-    b->staged = false;
-    for (const auto& in : b->insts) b->staged = b->staged || stagedWorkload(in->s, s0, b->ly);
+    b->staged = !b->insts.empty();
+    for (const auto& in : b->insts) b->staged = b->staged && stagedWorkload(in->s, s0, b->ly);
     const std::string key = ctltune::key(s0, b->ly, b->interp, first.C, first.R, b->staged) + "_batch";
     ctlk::LaunchShape sh = ctlk::defaultShape(b->staged);
     bool have = false;

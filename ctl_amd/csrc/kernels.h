@@ -61,7 +61,7 @@ struct LaunchShape {
     uint32_t block_waves_x = 2, block_waves_y = 2; // waves per workgroup
     uint32_t unroll = 4;
     uint32_t blocks_per_cu = 0; // 0: as many as registers allow; else capped with reserved LDS
-    // LDS staging (FP32 interleaved volumes, trilinear, one ray per pixel): samples per chunk
+    // LDS staging (interleaved volumes, trilinear, one ray per pixel): samples per chunk
     // (0 = unstaged) and LDS box capacity in texels (a multiple of 128)
     uint32_t chunk = 64;
     uint32_t lds_texels = 2560; // 20 KiB: seven workgroups per CU
@@ -103,10 +103,9 @@ struct RayParams {
 };
 
 // True if a projection with this storage, layout and mode can use the LDS-staged kernel.
-inline bool stageable(Storage st, Layout ly, const RayParams& rp)
+inline bool stageable(Storage, Layout ly, const RayParams& rp)
 {
-    return st == Storage::F32 && ly == Layout::Interleaved && rp.interpolate && rp.rpp_x == 1 &&
-           rp.rpp_y == 1;
+    return ly == Layout::Interleaved && rp.interpolate && rp.rpp_x == 1 && rp.rpp_y == 1;
 }
 
 struct Pitch {
@@ -149,8 +148,8 @@ struct BatchQueue {
 // Resets the queue to [0, total) split over the XCDs (a kernel, so it can sit in a graph).
 hipError_t launchQueueInit(BatchQueue* d_q, unsigned long long total, hipStream_t s);
 // Persistent kernel over `n_items` projections; all items share storage, layout and mode.
-// lds_texels > 0 with FP32 interleaved trilinear items selects the LDS-staged kernel with that box
-// (items with kc.chunk == 0 gather from HBM inside it); 0 selects the unstaged kernel.
+// lds_texels > 0 with interleaved trilinear items selects the LDS-staged kernel with that box
+// (every item must then have kc.chunk > 0); 0 selects the unstaged kernel.
 hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, Layout ly,
                        int interpolate, uint32_t unroll, uint32_t blocks_per_cu, uint32_t lds_texels,
                        uint32_t grid, BatchQueue* d_q, uint32_t stride, hipStream_t s);

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
 }
 
 // ---------------------------------------------------------------------------------------------
-// LDS-staged ray casting (FP32 interleaved volume, trilinear, one ray per pixel)
+// LDS-staged ray casting (interleaved volume, trilinear, one ray per pixel)
 //
 // The 256 rays of a workgroup walk their sample indices together in chunks of kc.chunk samples.
 // For each chunk the workgroup takes the exact box of texels that its rays' taps touch (sample
@@ -453,7 +453,8 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
 // of a gather through the vector L1, whose scattered cache lines bound the unstaged kernel. A
 // chunk whose box does not fit the LDS budget samples from HBM as the unstaged kernel does.
 // Samples, their order within a ray and the FP32 arithmetic of each sample are the same as in
-// castTile.
+// castTile. The box holds the volume's own 8-byte texels: FP32 z-pairs (two 16-byte LDS reads per
+// sample, rows y and y+1) or FP16/BF16 (y,z)-quads (one 16-byte read).
 
 // Wave-wide min / max: DPP reductions within each 16-lane row, then the four row results. All
 // 64 lanes must be active.
@@ -470,6 +471,27 @@ __device__ __forceinline__ int waveReduce(int v)
     return op(op(r0, r1), op(r2, r3));
 }
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Minimum of each 16-bit half (v_pk_min_u16).
+__device__ __forceinline__ uint32_t pkMin(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+
+// Wave-wide pkMin, as waveReduce. All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wavePkMin(uint32_t v)
+{
+    v = pkMin(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0xB1, 0xF, 0xF, false)));
+    v = pkMin(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x4E, 0xF, 0xF, false)));
+    v = pkMin(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x124, 0xF, 0xF, false)));
+    v = pkMin(v, uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), 0x128, 0xF, 0xF, false)));
+    const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return __builtin_amdgcn_readfirstlane(pkMin(pkMin(r0, r1), pkMin(r2, r3)));
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Trilinear sample from the LDS box (z-pair texels) at byte offset `ofs`, with rows px8 bytes
@@ -483,6 +505,20 @@ __device__ __forceinline__ float triLds(const float2* __restrict__ box, uint32_t
     const float2 b0 = q[0], b1 = q[1];
     const float c00 = lerp(a0.x, a1.x, wx), c01 = lerp(a0.y, a1.y, wx);
     const float c10 = lerp(b0.x, b1.x, wx), c11 = lerp(b0.y, b1.y, wx);
+    return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+}
+
+// FP16/BF16 quads: texels x and x+1 at byte offset `ofs`; the same lerps as QuadHalf::tri.
+template <Storage ST>
+__device__ __forceinline__ float quadLds(const float2* __restrict__ box, uint32_t ofs, float wx, float wy,
+                                         float wz)
+{
+    const uint2* p = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(box) + ofs);
+    const uint2 a = p[0], b = p[1];
+    const float c00 = lerpPair<ST>(a.x, false, b.x, false, wx); // y0 z0
+    const float c10 = lerpPair<ST>(a.x, true, b.x, true, wx);   // y1 z0
+    const float c01 = lerpPair<ST>(a.y, false, b.y, false, wx); // y0 z1
+    const float c11 = lerpPair<ST>(a.y, true, b.y, true, wx);   // y1 z1
     return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
 }
 
@@ -509,7 +545,7 @@ __device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, in
         if (live && uint32_t(i - a) <= span) part[0] += f(float(i));
 }
 
-template <int UNROLL>
+template <Storage ST, int UNROLL>
 __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                                const RayParams& rp, const CastConst& kc,
                                                float* __restrict__ out, uint32_t b, uint32_t wave,
@@ -553,61 +589,94 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             const int a = act ? max(c0, ray.i0) : INT_MAX;
             const int e = act ? min(c1, ray.i1) : INT_MIN;
             const bool live = a <= e;
-            // texel box of this lane's taps in the chunk: [lo, hi] per axis (z-pair texels carry
-            // z+1, so z needs no extra slice)
-            int bl[3], bh[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float qa = fmaf(float(a), ray.s[k], g.c[k]), qe = fmaf(float(e), ray.s[k], g.c[k]);
-                bl[k] = live ? int(fminf(qa, qe)) : INT_MAX;
-                bh[k] = live ? int(fmaxf(qa, qe)) + (k < 2 ? 1 : 0) : INT_MIN;
-            }
-            const int wlo = waveReduce<false>(live ? a : INT_MAX), whi = waveReduce<true>(live ? e : INT_MIN);
-            int wa = waveReduce<true>(live ? a : INT_MIN), we = waveReduce<false>(live ? e : INT_MAX);
-            if (wlo > whi || wa > we) { wa = whi + 1; we = whi; } // no shared range: all masked
+            // Texel box of this lane's taps in the chunk, [lo, hi] per axis (texels carry z+1, and
+            // y+1 for the FP16/BF16 quads, so only x and for FP32 y need one more), and the lane's
+            // sample range relative to c0. Reduced over the wave as packed 16-bit minima (a maximum
+            // m as the minimum of 0xffff - m): padded sizes stay below 65535 when a box is staged
+            // (kc.lds_texels > 0) and ranges below the chunk size. 0xffff marks "no live lane".
+            uint32_t pk[5];
             {
-                int r[6];
+                const int as = live ? a : c0, es = live ? e : c0; // in range for every lane
+                uint32_t lo[3], hi[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    r[k] = waveReduce<false>(bl[k]);
-                    r[3 + k] = waveReduce<true>(bh[k]);
+                    const float qa = fmaf(float(as), ray.s[k], g.c[k]), qe = fmaf(float(es), ray.s[k], g.c[k]);
+                    const uint32_t extra = (k == 0 || (k == 1 && ST == Storage::F32)) ? 1u : 0u;
+                    lo[k] = uint32_t(int(fminf(qa, qe))) & 0xffffu;
+                    hi[k] = 0xffffu - ((uint32_t(int(fmaxf(qa, qe))) + extra) & 0xffffu);
                 }
-                if (lane == 0)
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) red[parity][wave][k] = r[k];
+                const uint32_t ra = uint32_t(as - c0), re = uint32_t(es - c0);
+                pk[0] = live ? lo[0] | lo[1] << 16 : ~0u;
+                pk[1] = live ? hi[0] | hi[1] << 16 : ~0u;
+                pk[2] = live ? lo[2] | hi[2] << 16 : ~0u;
+                pk[3] = live ? ra | (0xffffu - ra) << 16 : ~0u;
+                pk[4] = live ? re | (0xffffu - re) << 16 : ~0u;
             }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) pk[k] = wavePkMin(pk[k]);
+            // the wave's sample range [wlo, whi] and the part [wa, we] all its live lanes share
+            int wlo = INT_MAX, whi = INT_MIN, wa = INT_MIN + 1, we = INT_MIN;
+            if ((pk[3] & 0xffffu) != 0xffffu) {
+                wlo = c0 + int(pk[3] & 0xffffu);
+                wa = c0 + int(0xffffu - (pk[3] >> 16));
+                we = c0 + int(pk[4] & 0xffffu);
+                whi = c0 + int(0xffffu - (pk[4] >> 16));
+                if (wa > we) { wa = whi + 1; we = whi; } // no shared range: all masked
+            }
+            if (lane == 0)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) red[parity][wave][k] = int(pk[k]);
             __syncthreads(); // the box of the previous chunk is no longer read; red[parity] is complete
-            int o[3], nb[3];
+            uint32_t bpk[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                o[k] = min(min(red[parity][0][k], red[parity][1][k]), min(red[parity][2][k], red[parity][3][k]));
-                const int h = max(max(red[parity][0][3 + k], red[parity][1][3 + k]),
-                                  max(red[parity][2][3 + k], red[parity][3][3 + k]));
-                nb[k] = o[k] == INT_MAX ? 0 : h - o[k] + 1; // positions lie in [0, n+1]: no overflow
-            }
-            if (nb[0] == 0) continue; // no ray of the workgroup samples in this chunk
+            for (int k = 0; k < 3; ++k)
+                bpk[k] = pkMin(pkMin(uint32_t(red[parity][0][k]), uint32_t(red[parity][1][k])),
+                               pkMin(uint32_t(red[parity][2][k]), uint32_t(red[parity][3][k])));
+            const int o[3] = { int(bpk[0] & 0xffffu), int(bpk[0] >> 16), int(bpk[2] & 0xffffu) };
+            const int nb[3] = { int(0xffffu - (bpk[1] & 0xffffu)) - o[0] + 1,
+                                int(0xffffu - (bpk[1] >> 16)) - o[1] + 1,
+                                int(0xffffu - (bpk[2] >> 16)) - o[2] + 1 };
+            // with a box to stage (16-bit coordinates valid), 0xffff marks a chunk in which no lane
+            // of the workgroup samples; without, sampling below finds no live lane
+            if (kc.lds_texels && o[0] == 0xffff) continue; // uniform
+            const bool small = kc.lds_texels && nb[0] >= 1 && nb[0] <= 256 && nb[1] >= 1 && nb[1] <= 256 &&
+                               nb[2] >= 1 && nb[2] <= 256;
             // LDS pitches: even rows (a 16-byte load never straddles two rows), not a multiple of
             // 16 texels (rows would share LDS banks)
             int px = (nb[0] + 1) & ~1;
             if ((px & 15) == 0) px += 2;
-            const bool small = nb[0] <= 256 && nb[1] <= 256 && nb[2] <= 256;
-            const int pz = px * nb[1];
-            const int total = small ? pz * nb[2] : INT_MAX;
-            const int n_ins = small ? (total + 127) >> 7 : INT_MAX; // 16-byte loads of 64 lanes = 128 texels
+            const int pz = small ? px * nb[1] : 0;
+            const int n_ins = small ? (pz * nb[2] + 127) >> 7 : 0; // 16-byte loads of 64 lanes = 128 texels
             if (small && n_ins * 128 <= int(kc.lds_texels)) {
-                const float rpx = 1.0f / float(px), rby = 1.0f / float(nb[1]);
-                for (int k = int(wave); k < n_ins; k += 4) {
-                    const int tt = k * 128 + 2 * int(lane);
-                    int row = int(float(tt) * rpx);
-                    int cx = tt - row * px;
+                // texel pairs t = 128 k + 2 lane of the box for k = wave, wave + 4, ...: position
+                // (cx, cy, cz) of the first by division, then steps of 512 texels with carries
+                int cx, cy, cz, dx, dy, dz;
+                {
+                    const float rpx = 1.0f / float(px), rby = 1.0f / float(nb[1]);
+                    const int t0 = int(wave) * 128 + 2 * int(lane);
+                    int row = int(float(t0) * rpx);
+                    cx = t0 - row * px;
                     if (cx < 0) { --row; cx += px; } else if (cx >= px) { ++row; cx -= px; }
-                    int cz = int(float(row) * rby);
-                    int cy = row - cz * nb[1];
+                    cz = int(float(row) * rby);
+                    cy = row - cz * nb[1];
                     if (cy < 0) { --cz; cy += nb[1]; } else if (cy >= nb[1]) { ++cz; cy -= nb[1]; }
+                    int drow = int(512.0f * rpx);
+                    dx = 512 - drow * px;
+                    if (dx < 0) { --drow; dx += px; } else if (dx >= px) { ++drow; dx -= px; }
+                    dz = int(float(drow) * rby);
+                    dy = drow - dz * nb[1];
+                    if (dy < 0) { --dz; dy += nb[1]; } else if (dy >= nb[1]) { ++dz; dy -= nb[1]; }
+                }
+                for (int k = int(wave); k < n_ins; k += 4) {
                     // texels past the box land in padding of the LDS box; past the volume they read 0
-                    const uint32_t src = (uint32_t(o[0] + cx) + uint32_t(o[1] + cy) * kc.pitch.ty +
+                    const uint32_t src = (uint32_t(o[0] + cx) + __umul24(uint32_t(o[1] + cy), kc.pitch.ty) +
                                           uint32_t(o[2] + cz) * kc.pitch.tz) << 3;
                     __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(box + k * 128), 16, src, 0, 0, 0);
+                    cx += dx;
+                    cy += dy;
+                    cz += dz;
+                    if (cx >= px) { cx -= px; ++cy; }
+                    if (cy >= nb[1]) { cy -= nb[1]; ++cz; }
                 }
                 __syncthreads(); // box loaded (the compiler waits for the loads before the barrier)
                 // byte offset of texel (ix, iy, iz) in the box: pitches in bytes, so that the address
@@ -620,12 +689,14 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     const float qz = fmaf(fi, ray.s[2], g.c[2]);
                     const uint32_t ofs = __umul24(uint32_t(int(qz)), pz8) +
                                          (__umul24(uint32_t(int(qy)), px8) + ((uint32_t(int(qx)) << 3) - base8));
-                    return triLds(box, ofs, px8, __builtin_amdgcn_fractf(qx), __builtin_amdgcn_fractf(qy),
-                                  __builtin_amdgcn_fractf(qz));
+                    const float wx = __builtin_amdgcn_fractf(qx), wy = __builtin_amdgcn_fractf(qy);
+                    const float wz = __builtin_amdgcn_fractf(qz);
+                    if constexpr (ST == Storage::F32) return triLds(box, ofs, px8, wx, wy, wz);
+                    else return quadLds<ST>(box, ofs, wx, wy, wz);
                 });
             } else {
                 sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, [&](float fi) {
-                    return sampleAt<Storage::F32, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
+                    return sampleAt<ST, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
                 });
             }
         }
@@ -636,14 +707,14 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
         out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
 }
 
-template <int UNROLL>
+template <Storage ST, int UNROLL>
 __global__ __launch_bounds__(kBlock) void staged_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
                                                         RayParams rp, CastConst kc, float* __restrict__ out)
 {
     extern __shared__ __attribute__((aligned(16))) float2 s_box[];
     __shared__ int s_red[2][4][8];
     static_assert(sizeof(s_red) <= kStaticLds, "static LDS beyond the reserve");
-    castTileStaged<UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
+    castTileStaged<ST, UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
                            s_box, s_red);
 }
 
@@ -709,10 +780,9 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem* __restri
     }
 }
 
-// The same queue over FP32 interleaved trilinear items: tiles of items with kc.chunk > 0 run the
-// LDS-staged caster (kc.lds_texels must not exceed the launch's dynamic LDS), the others gather
-// from HBM.
-template <int UNROLL>
+// The same queue over interleaved trilinear items that all have kc.chunk > 0, cast with the
+// LDS-staged caster (kc.lds_texels must not exceed the launch's dynamic LDS).
+template <Storage ST, int UNROLL>
 __global__ __launch_bounds__(kBlock) void batch_staged_kernel(const BatchItem* __restrict__ items,
                                                               uint32_t n_items, BatchQueue* __restrict__ q,
                                                               uint32_t stride)
@@ -729,12 +799,8 @@ __global__ __launch_bounds__(kBlock) void batch_staged_kernel(const BatchItem* _
         grabTile(items, n_items, q, stride, home, &s_tile, &s_item, &t, &k);
         if (t == ~0ull) break;
         const BatchItem& it = items[k];
-        const uint32_t tile = uint32_t(t * stride - it.tile_begin);
-        if (it.kc.chunk)
-            castTileStaged<UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out, tile, threadIdx.x >> 6, s_box, s_red);
-        else
-            castTile<Storage::F32, Layout::Interleaved, true, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out,
-                                                                      tile, threadIdx.x >> 6);
+        castTileStaged<ST, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out, uint32_t(t * stride - it.tile_begin),
+                                   threadIdx.x >> 6, s_box, s_red);
     }
 }
 
@@ -1035,7 +1101,10 @@ CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayPa
     }
     kc.blo = rp.interpolate ? 0.0f : 1.0f;
     kc.chunk = (vol && stageable(vol->storage, vol->layout, rp)) ? sh.chunk : 0u;
-    kc.lds_texels = sh.lds_texels;
+    // the staged kernel packs texel coordinates into 16 bits: larger volumes chunk their samples
+    // but never stage a box
+    const bool fits16 = vol && vol->n[0] + 2 < 0xffffu && vol->n[1] + 2 < 0xffffu && vol->n[2] + 2 < 0xffffu;
+    kc.lds_texels = fits16 ? sh.lds_texels : 0u;
     if (vol) {
         const uint32_t eb = uint32_t(texelBytes(vol->storage, vol->layout));
         kc.pitch.ty = vol->pitch_y;
@@ -1058,6 +1127,18 @@ hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams&
     case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     case 4: raycast_kernel<ST, LY, INTERP, 4><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     default: raycast_kernel<ST, LY, INTERP, 2><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    }
+    return hipGetLastError();
+}
+
+template <Storage ST>
+hipError_t launchStaged(const VolumeView& vol, const RayGeom* g, const RayParams& rp, const CastConst& kc,
+                        uint32_t unroll, uint32_t blocks, uint32_t lds, float* out, hipStream_t s)
+{
+    switch (unroll) {
+    case 1: staged_kernel<ST, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    case 4: staged_kernel<ST, 4><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    default: staged_kernel<ST, 2><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     }
     return hipGetLastError();
 }
@@ -1130,15 +1211,15 @@ hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayG
     if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
     const uint32_t lds = ldsForBlocks(shape.blocks_per_cu);
     if (kc.chunk) {
-        if (kc.lds_texels < 1024 || kc.lds_texels > kMaxLdsTexels || kc.lds_texels % 128) return hipErrorInvalidValue;
+        if (kc.chunk < 16 || kc.chunk > 1024 || kc.lds_texels > kMaxLdsTexels || kc.lds_texels % 128)
+            return hipErrorInvalidValue;
         const uint32_t box = kc.lds_texels * 8;
         const uint32_t dyn = box > lds ? box : lds;
-        switch (shape.unroll) {
-        case 1: staged_kernel<1><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
-        case 4: staged_kernel<4><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
-        default: staged_kernel<2><<<uint32_t(blocks), kBlock, dyn, s>>>(vol, d_geom, rp, kc, d_out); break;
+        switch (vol.storage) {
+        case Storage::F16: return launchStaged<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), dyn, d_out, s);
+        case Storage::BF16: return launchStaged<Storage::BF16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), dyn, d_out, s);
+        default: return launchStaged<Storage::F32>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), dyn, d_out, s);
         }
-        return hipGetLastError();
     }
     switch (vol.storage) {
     case Storage::F16: return launchCastI<Storage::F16>(vol, d_geom, rp, kc, shape.unroll, uint32_t(blocks), lds, d_out, s);
@@ -1224,12 +1305,22 @@ const void* batchKernelS(Layout ly, int interp, uint32_t unroll)
     return interp ? batchKernelU<ST, Layout::Plain, true>(unroll) : batchKernelU<ST, Layout::Plain, false>(unroll);
 }
 
-const void* batchStagedKernel(uint32_t unroll)
+template <Storage ST>
+const void* batchStagedKernelU(uint32_t unroll)
 {
     switch (unroll) {
-    case 1: return reinterpret_cast<const void*>(&batch_staged_kernel<1>);
-    case 4: return reinterpret_cast<const void*>(&batch_staged_kernel<4>);
-    default: return reinterpret_cast<const void*>(&batch_staged_kernel<2>);
+    case 1: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 1>);
+    case 4: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 4>);
+    default: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 2>);
+    }
+}
+
+const void* batchStagedKernel(Storage st, uint32_t unroll)
+{
+    switch (st) {
+    case Storage::F16: return batchStagedKernelU<Storage::F16>(unroll);
+    case Storage::BF16: return batchStagedKernelU<Storage::BF16>(unroll);
+    default: return batchStagedKernelU<Storage::F32>(unroll);
     }
 }
 
@@ -1249,9 +1340,9 @@ uint32_t batchLds(uint32_t lds_texels, uint32_t blocks_per_cu)
     return box > cap ? box : cap;
 }
 
-bool batchStaged(Storage st, Layout ly, int interp, uint32_t lds_texels)
+bool batchStaged(Storage, Layout ly, int interp, uint32_t lds_texels)
 {
-    return lds_texels && st == Storage::F32 && ly == Layout::Interleaved && interp;
+    return lds_texels && ly == Layout::Interleaved && interp;
 }
 
 } // namespace
@@ -1266,7 +1357,7 @@ hipError_t batchGrid(Storage st, Layout ly, int interpolate, uint32_t unroll, ui
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, staged ? batchStagedKernel(unroll) : batchKernel(st, ly, interpolate, unroll), kBlock,
+            &per_cu, staged ? batchStagedKernel(st, unroll) : batchKernel(st, ly, interpolate, unroll), kBlock,
             staged ? batchLds(lds_texels, blocks_per_cu) : ldsForBlocks(blocks_per_cu));
     if (e != hipSuccess) return e;
     *grid = uint32_t(cus * (per_cu > 0 ? per_cu : 1));
@@ -1280,7 +1371,7 @@ hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, L
     if (n_items == 0 || grid == 0) return hipSuccess;
     if (lds_texels > kMaxLdsTexels || lds_texels % 128) return hipErrorInvalidValue;
     const bool staged = batchStaged(st, ly, interpolate, lds_texels);
-    const void* fn = staged ? batchStagedKernel(unroll) : batchKernel(st, ly, interpolate, unroll);
+    const void* fn = staged ? batchStagedKernel(st, unroll) : batchKernel(st, ly, interpolate, unroll);
     void* args[] = { const_cast<BatchItem**>(&d_items), &n_items, &d_q, &stride };
     return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args,
                            staged ? batchLds(lds_texels, blocks_per_cu) : ldsForBlocks(blocks_per_cu), s);

@@ -124,7 +124,7 @@ struct Slot {
     bool busy = false;
 };
 
-using GroupKey = std::tuple<int, int, int>; // storage, layout, interpolate
+using GroupKey = std::tuple<int, int, int, int>; // storage, layout, interpolate, LDS-staged
 
 double percentile(std::vector<double> v, double q)
 {
@@ -235,7 +235,8 @@ ctlproj_status prepare(ctlproj_server* S, Request* r)
 
 GroupKey groupOf(const Request* r)
 {
-    return GroupKey(int(r->vol->view.storage), int(r->vol->view.layout), r->s.interpolate ? 1 : 0);
+    return GroupKey(int(r->vol->view.storage), int(r->vol->view.layout), r->s.interpolate ? 1 : 0,
+                    stagedWorkload(r->s, r->vol->view.storage, r->vol->view.layout) ? 1 : 0);
 }
 
 // Admission in arrival order; a request that does not fit lets later, smaller ones pass.
@@ -290,10 +291,9 @@ void failChunksLocked(ctlproj_server* S, std::vector<Chunk>& chunks, ctlproj_sta
 void launchLocked(ctlproj_server* S, Slot& slot)
 {
     const GroupKey key = groupOf(S->ready.front().r);
-    // FP32 interleaved trilinear groups run the LDS-staged kernel (its requests with more than one
-    // ray per pixel gather from HBM inside it); the others the unstaged kernel
-    const bool staged = std::get<0>(key) == int(ctlk::Storage::F32) &&
-                        std::get<1>(key) == int(ctlk::Layout::Interleaved) && std::get<2>(key);
+    // groups of trilinear requests with one ray per pixel run the LDS-staged kernel, the others
+    // the unstaged kernel
+    const bool staged = std::get<3>(key) != 0;
     const ctlk::LaunchShape shape = ctlk::defaultShape(staged);
     const uint32_t lds_texels = staged ? shape.lds_texels : 0u;
     uint64_t pixels = 0;

@@ -58,8 +58,9 @@ typedef struct ctlproj_settings {
     uint32_t tile_h;   /* pixels per wavefront tile along rows (0 = auto) */
     uint32_t unroll;   /* ray-step unroll of the sampling loop (0 = auto) */
     uint32_t blocks_per_cu; /* cap on resident workgroups per CU, 1..8 (0 = auto) */
-    int32_t lds_chunk;      /* LDS staging of FP32 trilinear projections: samples per chunk,
-                               16..1024 (0 = auto, -1 = off: gather from HBM through L1) */
+    int32_t lds_chunk;      /* LDS staging of trilinear projections with one ray per pixel:
+                               samples per chunk, 16..1024 (0 = auto: on for FP32 storage, off
+                               for FP16/BF16; -1 = off: gather from HBM through the L1) */
     uint32_t lds_texels;    /* LDS box capacity in 8-byte texels, 1024..20352 (0 = auto) */
 } ctlproj_settings;
 

@@ -106,6 +106,20 @@ def test_lds_staged_matches_unstaged_and_ref32(small, small_refs, chunk, texels)
     assert e <= L2_F32 and m <= MAR_F32
 
 
+@pytest.mark.parametrize("storage,bar", [("f16", L2_F16), ("bf16", L2_BF16)])
+def test_lds_staged_half_storage(small, small_refs, storage, bar):
+    """FP16/BF16 (y,z)-quad texels staged in LDS: against the unstaged kernel with the same tile and
+    unroll, and against ref64."""
+    r64 = small_refs[2][::5]
+    pm = small.pmats[::5]
+    plain, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
+    for chunk, texels in ((64, 0), (16, 1024)):
+        got, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8), unroll=4)
+        d, e = oracle.rel_l2(plain, got), oracle.rel_l2(r64, got)
+        print(f"lds {storage} chunk {chunk} texels {texels}: vs unstaged {d:.3e}, vs ref64 {e:.3e}")
+        assert d <= 1e-6 and e <= bar
+
+
 def test_lds_staged_real_size_oblique_views():
     """Full-size geometry: views at 0, 45, 137 and 225 degrees (boxes of axis-aligned and
     diagonal ray bundles), staged against unstaged and ref32."""

@@ -24,10 +24,15 @@ HBM_PEAK_GBPS = 8000.0
 # stream probe (coalesced 16-byte loads of an L1-resident window) reaches 98% of it.
 L1_PEAK_GBPS = 256 * 2.4 * 64.0
 
-# HBM traffic per launch of the 360-view FP32 projection, from the rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE passes in profiles/raycast_f32_256cube_360views_1024x768.md, in bytes
-# (KiB x 1024, uncalibrated).
-TRAFFIC_F32_BYTES = (4.5377e7 + 1.299e6) * 1024
+# HBM traffic per launch of the 360-view FP32 projection (LDS-staged kernel), from the rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_f32_256cube_360views_1024x768.md, in
+# KiB per dispatch, each scaled by what the counter reads for a plain 1 GiB device copy
+# (bench.py --copy-calibration: FETCH_SIZE 524300 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
+FETCH_KIB, FETCH_CAL = 5.0576e7, 1048576 / 524300
+WRITE_KIB, WRITE_CAL = 1.1383e6, 1.0
+TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
+# LDS read bandwidth of gfx950: 256 CUs x 2.4 GHz x 128 B/clk.
+LDS_PEAK_GBPS = 256 * 2.4 * 128.0
 
 
 def log(msg: str) -> None:
@@ -54,6 +59,8 @@ def bench_real(args, torch, W, ctl, oracle):
     log(f"workload: {w.name}")
     p = ctl.RayCasterProjector()
     p.settings().raySampling = w.ray_sampling
+    if args.no_lds:
+        p.settings().lds_chunk = -1
     p.configure((w.pmats, w.channels, w.rows))
     out = torch.empty(w.views * w.modules * w.rows * w.channels, dtype=torch.float32, device="cuda")
     res = {}
@@ -66,6 +73,10 @@ def bench_real(args, torch, W, ctl, oracle):
                             gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall,
                             kernel_gsamples_s=n_samples / (kms * 1e-3) / 1e9,
                             gather_GBps=n_samples * 8 * eb / (kms * 1e-3) / 1e9)
+        if args.kernels_only:
+            log(f"real {storage}: {json.dumps(res[storage])}")
+            del vol
+            continue
         # gather ceiling of this access pattern: the same loads without the interpolation
         pg = [p.gather_probe(vol, "gather", out.data_ptr(), out.numel()) for _ in range(4)][1:]
         ceil = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in pg]))
@@ -76,6 +87,8 @@ def bench_real(args, torch, W, ctl, oracle):
             res[storage]["probe_l1_stream_GBps"] = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in ps]))
         log(f"real {storage}: {json.dumps(res[storage])}")
         del vol
+    if args.kernels_only:
+        return w, res
     if not args.no_errors:
         pick = [0, 137]
         r32, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows,
@@ -287,7 +300,23 @@ def main() -> int:
     ap.add_argument("--no-errors", action="store_true", help="skip the oracle error statements")
     ap.add_argument("--only-real", action="store_true", help="only the 256^3 / 360-view workload")
     ap.add_argument("--quick", action="store_true", help="fewer served requests")
+    ap.add_argument("--no-lds", action="store_true",
+                    help="real workload with the gathers from HBM through L1 (no LDS staging)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="with --only-real: one timed projection per variant and nothing else (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (calibrates the FETCH_SIZE / "
+                         "WRITE_SIZE counters behind roofline.traffic) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": src.numel() * 4, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported here. The "
               "projector shards over views with no collective (see DESIGN.md section 7).",
@@ -312,7 +341,7 @@ def main() -> int:
     if cpu:
         log(f"cpu baseline: {cpu}")
 
-    f32 = real["f32"]
+    f32 = real["f32"] if "f32" in real else next(iter(real.values()))  # headline: FP32 unless not run
     achieved = f32["gather_GBps"]
     line = {
         "metric": "Gsamples/s & views/s at 256³/360 views vs plain FP32; served p50/p99 latency",
@@ -327,10 +356,13 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": TRAFFIC_F32_BYTES,
                      "algorithmic_bytes_per_sample": 32, "samples_per_launch": f32["n_samples"],
                      "kernel_ms": f32["kernel_ms"],
-                     "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample); they "
-                             "are served from L1/L2, traffic = HBM bytes per launch. The on-chip "
-                             "peak is the vector L1 data path (l1_gather_peak, 64 B/clk/CU); "
-                             "probe_gather_GBps = the same loads without interpolation",
+                     "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample). The "
+                             "staged kernel serves them from LDS boxes filled once per 64-sample "
+                             "chunk; traffic = calibrated HBM/fabric bytes per launch. On-chip "
+                             "peaks: LDS 128 B/clk/CU (lds_peak), vector L1 64 B/clk/CU "
+                             "(l1_gather_peak, the ceiling of gathering without LDS). The staged "
+                             "kernel is bound by VALU issue (about 38 VALU instructions per sample)",
+                     "lds_peak": LDS_PEAK_GBPS, "lds_frac": achieved / LDS_PEAK_GBPS,
                      "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS,
                      "probe_gather_GBps": f32.get("probe_gather_GBps"),
                      "frac_of_probe_gather": f32.get("frac_of_probe_gather"),

@@ -129,6 +129,7 @@ struct Ray {
 __device__ __forceinline__ void checkFace(float h0, float h1, float c10, float c11, float c20,
                                           float c21, float lambda, float& lo, float& hi)
 {
+#pragma clang fp contract(off)
     const bool in = (h0 >= c10) & (h1 >= c11) & (h0 <= c20) & (h1 <= c21);
     lo = (in & (lambda < lo)) ? lambda : lo;
     hi = (in & (lambda > hi)) ? lambda : hi;
@@ -137,9 +138,13 @@ __device__ __forceinline__ void checkFace(float h0, float h1, float c10, float c
 // Direction, step and the reference's sample range for the sub-ray through detector
 // coordinate (px, py). The slab test restates calculateIntersections
 // (raycasterprojector_interp.cl:88-134) in the texel frame whose box is [blo, n + 1].
+// Multiply-adds of the ray setup are not fused (fp contract off, as in the oracle): otherwise the
+// compiler may fuse them differently in different kernels, and a grazing ray would gain or lose a
+// sample at its ends depending on which kernel casts it.
 __device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, float inc,
                                         const float vs[3], const float nhi[3], float blo)
 {
+#pragma clang fp contract(off)
     Ray r;
     const float ex = px - g.xr, ey = py - g.yr;
     float d[3];
@@ -184,6 +189,7 @@ __device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, fl
 __device__ __forceinline__ bool insideQ(int i, const float s[3], const float c[3], float blo,
                                         const float nhi[3])
 {
+#pragma clang fp contract(off)
     const float fi = float(i);
     bool in = true;
 #pragma unroll
@@ -699,9 +705,16 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     return sampleAt<ST, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
                 });
             }
-        }
+            // two-level summation: the chunk's partial sums (at most kc.chunk samples) are added
+            // to the ray's total once per chunk, which keeps the FP32 rounding of long rays small
+            float chunk_sum = 0.0f;
 #pragma unroll
-        for (int u = 0; u < UNROLL; ++u) acc += part[u];
+            for (int u = 0; u < UNROLL; ++u) {
+                chunk_sum += part[u];
+                part[u] = 0.0f;
+            }
+            acc += chunk_sum;
+        }
     }
     if (pixel)
         out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);

@@ -90,19 +90,30 @@ def test_unroll_and_tile_variants_agree(small, small_refs):
             assert e <= L2_F32, (tile, unroll, e)
 
 
+# Staged vs unstaged kernel: the same samples with the same per-sample arithmetic, summed in a
+# different order (per-chunk sums vs unrolled partial sums). Summation order moves a pixel by far
+# less than SAME_SAMPLES; one extra, missing or wrong tap moves it by about 1/(samples per ray),
+# at least 5e-4 at these sizes.
+SAME_SAMPLES = 2e-4
+
+
+def check_same_samples(plain, got, what):
+    d, dm = oracle.rel_l2(plain, got), oracle.max_abs_rel(plain, got)
+    print(f"{what}: vs unstaged rel L2 {d:.3e}, max abs rel {dm:.3e}")
+    assert dm <= SAME_SAMPLES, (what, dm)
+
+
 @pytest.mark.parametrize("chunk,texels", [(16, 0), (64, 0), (256, 0), (64, 1024), (1024, 20352)])
 def test_lds_staged_matches_unstaged_and_ref32(small, small_refs, chunk, texels):
-    """FP32 trilinear projections stage the taps of each chunk of samples in LDS. Against the
-    unstaged kernel with the same tile and unroll only the split of the partial sums at chunk
-    boundaries differs; 1024 texels forces the HBM fallback for the larger boxes."""
+    """FP32 trilinear projections stage the taps of each chunk of samples in LDS; 1024 texels
+    forces the HBM fallback for the larger boxes."""
     r32 = small_refs[0]
     pm = small.pmats[::5]
     plain, _, _ = run_gpu(small, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
     got, _, _ = run_gpu(small, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8), unroll=4)
-    d = oracle.rel_l2(plain, got)
+    check_same_samples(plain, got, f"lds chunk {chunk} texels {texels}")
     e, m = oracle.rel_l2(r32[::5], got), oracle.max_abs_rel(r32[::5], got)
-    print(f"lds chunk {chunk} texels {texels}: vs unstaged {d:.3e}, vs ref32 {e:.3e} / {m:.3e}")
-    assert d <= 1e-6
+    print(f"lds chunk {chunk} texels {texels}: vs ref32 {e:.3e} / {m:.3e}")
     assert e <= L2_F32 and m <= MAR_F32
 
 
@@ -115,9 +126,10 @@ def test_lds_staged_half_storage(small, small_refs, storage, bar):
     plain, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
     for chunk, texels in ((64, 0), (16, 1024)):
         got, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8), unroll=4)
-        d, e = oracle.rel_l2(plain, got), oracle.rel_l2(r64, got)
-        print(f"lds {storage} chunk {chunk} texels {texels}: vs unstaged {d:.3e}, vs ref64 {e:.3e}")
-        assert d <= 1e-6 and e <= bar
+        check_same_samples(plain, got, f"lds {storage} chunk {chunk} texels {texels}")
+        e = oracle.rel_l2(r64, got)
+        print(f"lds {storage} chunk {chunk} texels {texels}: vs ref64 {e:.3e}")
+        assert e <= bar
 
 
 def test_lds_staged_real_size_oblique_views():
@@ -128,10 +140,10 @@ def test_lds_staged_real_size_oblique_views():
     r32, _ = run_ref(w, views=pick)
     plain, _, _ = run_gpu(w, pm=w.pmats[pick], lds_chunk=-1, tile=(16, 4), unroll=2)
     got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 2
-    d = oracle.rel_l2(plain, got)
+    check_same_samples(plain, got, f"lds real views {pick}")
     e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
-    print(f"lds real views {pick}: vs unstaged {d:.3e}, vs ref32 {e:.3e} / {m:.3e}")
-    assert d <= 1e-6
+    e0 = oracle.rel_l2(r32, plain)
+    print(f"lds real views {pick}: vs ref32 {e:.3e} / {m:.3e} (unstaged {e0:.3e})")
     assert e <= L2_F32 and m <= MAR_F32
 
 

@@ -116,6 +116,7 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_phantom_ellipsoids": (_i, [_u32p, _f32p, _u32, C.c_uint64, C.c_float, C.c_float,
                                         C.c_float, C.c_float, C.c_float, _f32p]),
     "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
+    "ctlproj_settings_optimized_for": (_i, [_f32p, C.c_double, C.c_double, C.POINTER(Settings)]),
     "ctlproj_diag_gather_probe": (_i, [_vp, _vp, _i, _vp, _sz, _f32p, _f64p]),
     "ctlproj_projector_autotune": (_i, [_vp, _vp, _f32p]),
     "ctlproj_autotune_table": (_i, [C.c_char_p, _sz, C.POINTER(_sz)]),

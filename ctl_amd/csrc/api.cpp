@@ -332,6 +332,38 @@ void ctlproj_settings_default(ctlproj_settings* s)
     s->interpolate = 1;
 }
 
+ctlproj_status ctlproj_settings_optimized_for(const float voxel_mm[3], double pixel_width_mm,
+                                              double pixel_height_mm, ctlproj_settings* out)
+{
+    if (!voxel_mm || !out) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (pixel_width_mm == 0.0 || pixel_height_mm == 0.0)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "pixel dimensions are singular");
+    ctlproj_settings_default(out);
+    // the preset ray grid whose aspect is closest to the pixel's (first one wins on ties), laid
+    // with its longer side along the pixel's longer side
+    double ratio = pixel_width_mm / pixel_height_mm;
+    const bool wide = ratio > 1.0;
+    if (wide) ratio = 1.0 / ratio;
+    static const uint32_t presets[5][2] = { { 1, 2 }, { 1, 3 }, { 1, 4 }, { 2, 3 }, { 3, 4 } };
+    double best = std::fabs(ratio - 1.0);
+    for (const auto& r : presets) {
+        const double dev = std::fabs(ratio - double(r[0]) / double(r[1]));
+        if (dev < best) {
+            best = dev;
+            out->rays_per_pixel[0] = wide ? r[1] : r[0];
+            out->rays_per_pixel[1] = wide ? r[0] : r[1];
+        }
+    }
+    const double voxel = double(std::min(voxel_mm[0], std::min(voxel_mm[1], voxel_mm[2])));
+    const double sub_pixel = std::min(pixel_width_mm / out->rays_per_pixel[0],
+                                      pixel_height_mm / out->rays_per_pixel[1]);
+    out->volume_upsampling = std::max(uint32_t(voxel / sub_pixel), 1u); // voxels larger than a sub-pixel
+    const uint32_t more = std::max(uint32_t(sub_pixel / voxel), 1u);    // voxels smaller than a sub-pixel
+    out->rays_per_pixel[0] *= more;
+    out->rays_per_pixel[1] *= more;
+    return CTLPROJ_OK;
+}
+
 const char* ctlproj_last_error(void) { return lastError(); }
 
 ctlproj_status ctlproj_device_count(int* count)

@@ -335,6 +335,17 @@ class Settings:
         s.lds_texels = self.lds_texels
         return s
 
+    @staticmethod
+    def optimizedFor(volume: "VoxelVolume", detector) -> "Settings":
+        """Settings::optimizedFor (raycasterprojector.cpp:482-521): rays per pixel and volume
+        upsampling from the detector's pixel size and the volume's smallest voxel size."""
+        c = _lib.Settings()
+        vox = (C.c_float * 3)(*volume.voxel_size)
+        w, h = detector.pixelDimensions
+        check(lib().ctlproj_settings_optimized_for(vox, float(w), float(h), C.byref(c)))
+        return Settings(raysPerPixel=list(c.rays_per_pixel), raySampling=c.ray_sampling,
+                        volumeUpSampling=c.volume_upsampling, interpolate=bool(c.interpolate))
+
 
 class RayCasterProjector:
     """Drop-in for CTL::OCL::RayCasterProjector (raycasterprojector.h:20-58)."""
@@ -350,6 +361,31 @@ class RayCasterProjector:
 
     def settings(self) -> Settings:
         return self._settings
+
+    # SerializationInterface (raycasterprojector.cpp:60-91, abstractprojector.h:114-123,
+    # serializationinterface.h:83-88): the same keys and type id as the reference
+    TYPE_ID = 1  # CTL_TYPE_ID(1), raycasterprojector.h:22
+
+    def parameter(self) -> dict:
+        s = self._settings
+        return {"Rays per pixel X": int(s.raysPerPixel[0]), "Rays per pixel Y": int(s.raysPerPixel[1]),
+                "Ray sampling step length": float(s.raySampling),
+                "Volume upsampling factor": int(s.volumeUpSampling), "Interpolate": bool(s.interpolate)}
+
+    def setParameter(self, parameter: dict) -> None:
+        s = self._settings
+        s.raysPerPixel = [int(parameter.get("Rays per pixel X", 1)), int(parameter.get("Rays per pixel Y", 1))]
+        s.raySampling = float(parameter.get("Ray sampling step length", 0.3))
+        s.volumeUpSampling = int(parameter.get("Volume upsampling factor", 1))
+        s.interpolate = bool(parameter.get("Interpolate", True))
+
+    def toVariant(self) -> dict:
+        return {"type-id": self.TYPE_ID, "parameters": self.parameter(), "#": "RayCasterProjector"}
+
+    def fromVariant(self, variant: dict) -> None:
+        if variant.get("type-id") != self.TYPE_ID:
+            raise ValueError("variant does not describe a RayCasterProjector")
+        self.setParameter(variant.get("parameters", {}))
 
     def _push_settings(self) -> None:
         s = self._settings._c()

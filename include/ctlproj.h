@@ -75,6 +75,15 @@ typedef void (*ctlproj_progress_fn)(int view, void* user);
 /* ---- environment (replaces OpenCLConfig, openclconfig.cpp:15-171) ----------------------- */
 
 void ctlproj_settings_default(ctlproj_settings* s);
+
+/* RayCasterProjector::Settings::optimizedFor (raycasterprojector.cpp:482-521): rays per pixel from
+ * the pixel aspect ratio (presets 1:2, 1:3, 1:4, 2:3, 3:4), a volume upsampling factor for voxels
+ * larger than a sub-pixel, and more rays for voxels smaller than a sub-pixel; other fields take
+ * their defaults. voxel_mm: the volume's voxel size (its smallest component counts); pixel sizes
+ * of the detector in mm. A zero pixel size returns CTLPROJ_ERR_INVALID_ARGUMENT (the reference
+ * throws). Needs no device. */
+ctlproj_status ctlproj_settings_optimized_for(const float voxel_mm[3], double pixel_width_mm,
+                                              double pixel_height_mm, ctlproj_settings* out);
 const char* ctlproj_last_error(void);
 ctlproj_status ctlproj_device_count(int* count);
 ctlproj_status ctlproj_context_create(int device, ctlproj_context** out);

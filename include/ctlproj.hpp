@@ -17,6 +17,7 @@
 #include <functional>
 #include <memory>
 #include <stdexcept>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -96,7 +97,40 @@ public:
         float raySampling = 0.3f;
         uint32_t volumeUpSampling = 1;
         bool interpolate = true;
+
+        // Settings::optimizedFor (raycasterprojector.cpp:482-521)
+        static Settings optimizedFor(const VoxelVolume& volume, double pixelWidthMm, double pixelHeightMm)
+        {
+            ctlproj_settings c;
+            check(ctlproj_settings_optimized_for(volume.voxelSize.data(), pixelWidthMm, pixelHeightMm, &c));
+            Settings s;
+            s.raysPerPixel[0] = c.rays_per_pixel[0];
+            s.raysPerPixel[1] = c.rays_per_pixel[1];
+            s.raySampling = c.ray_sampling;
+            s.volumeUpSampling = c.volume_upsampling;
+            s.interpolate = c.interpolate != 0;
+            return s;
+        }
     };
+
+    // parameter() / setParameter() (raycasterprojector.cpp:69-91): the reference's keys
+    std::map<std::string, double> parameter() const
+    {
+        return { { "Rays per pixel X", settings_.raysPerPixel[0] },
+                 { "Rays per pixel Y", settings_.raysPerPixel[1] },
+                 { "Ray sampling step length", settings_.raySampling },
+                 { "Volume upsampling factor", settings_.volumeUpSampling },
+                 { "Interpolate", settings_.interpolate ? 1.0 : 0.0 } };
+    }
+    void setParameter(const std::map<std::string, double>& p)
+    {
+        auto get = [&](const char* k, double d) { auto it = p.find(k); return it == p.end() ? d : it->second; };
+        settings_.raysPerPixel[0] = uint32_t(get("Rays per pixel X", 1));
+        settings_.raysPerPixel[1] = uint32_t(get("Rays per pixel Y", 1));
+        settings_.raySampling = float(get("Ray sampling step length", 0.3));
+        settings_.volumeUpSampling = uint32_t(get("Volume upsampling factor", 1));
+        settings_.interpolate = get("Interpolate", 1.0) != 0.0;
+    }
 
     explicit RayCasterProjector(std::shared_ptr<Context> ctx = std::make_shared<Context>(0))
         : ctx_(std::move(ctx))

@@ -294,3 +294,27 @@ def test_cpp_mirror_program(tmp_path):
     e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
     print(f"C++ mirror: rel L2 {e:.3e}, max abs rel {m:.3e}")
     assert e <= L2_F32 and m <= MAR_F32
+
+
+def test_parameter_serialization_round_trip():
+    """parameter()/setParameter() and toVariant()/fromVariant() keep the reference's keys and type
+    id (raycasterprojector.cpp:60-91, abstractprojector.h:114-123)."""
+    from ctl_amd import FlatPanelDetector, Settings
+    p = RayCasterProjector()
+    s = p.settings()
+    s.raysPerPixel, s.raySampling, s.volumeUpSampling, s.interpolate = [2, 3], 0.2, 2, False
+    v = p.toVariant()
+    assert v["type-id"] == 1 and v["#"] == "RayCasterProjector"
+    assert v["parameters"] == {"Rays per pixel X": 2, "Rays per pixel Y": 3,
+                               "Ray sampling step length": pytest.approx(0.2),
+                               "Volume upsampling factor": 2, "Interpolate": False}
+    q = RayCasterProjector()
+    q.fromVariant(v)
+    assert q.parameter() == p.parameter()
+    q.setParameter({})  # missing keys take the reference defaults
+    assert q.parameter() == {"Rays per pixel X": 1, "Rays per pixel Y": 1,
+                             "Ray sampling step length": pytest.approx(0.3),
+                             "Volume upsampling factor": 1, "Interpolate": True}
+    o = Settings.optimizedFor(VoxelVolume(np.zeros((4, 4, 4), np.float32), (0.25, 0.25, 0.25)),
+                              FlatPanelDetector((16, 16), (1.0, 2.0)))
+    assert o.raysPerPixel == [4, 8] and o.volumeUpSampling == 1

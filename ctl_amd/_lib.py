@@ -20,6 +20,7 @@ ERR_NO_DATA = 6
 ERR_OVER_BUDGET = 7
 ERR_INJECTED = 8
 ERR_TIMEOUT = 9
+ERR_IO = 10
 
 F32, F16, BF16 = 0, 1, 2
 CARM, TUBULAR = 0, 1
@@ -38,6 +39,24 @@ class Settings(C.Structure):
         ("blocks_per_cu", C.c_uint32),
         ("lds_chunk", C.c_int32),
         ("lds_texels", C.c_uint32),
+    ]
+
+
+class NrrdInfo(C.Structure):
+    _fields_ = [
+        ("version", C.c_uint32),
+        ("ndim", C.c_uint32),
+        ("nsizes", C.c_uint32),
+        ("elem", C.c_int32),
+        ("elem_bytes", C.c_uint32),
+        ("sizes", C.c_uint64 * 4),
+        ("count", C.c_uint64),
+        ("header_bytes", C.c_uint64),
+        ("has_spacing", C.c_int32),
+        ("spacing", C.c_double * 3),
+        ("has_origin", C.c_int32),
+        ("origin", C.c_double * 3),
+        ("data_complete", C.c_int32),
     ]
 
 
@@ -117,6 +136,13 @@ SIGNATURES: dict[str, tuple] = {
                                         C.c_float, C.c_float, C.c_float, _f32p]),
     "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
     "ctlproj_settings_optimized_for": (_i, [_f32p, C.c_double, C.c_double, C.POINTER(Settings)]),
+    "ctlproj_den_info": (_i, [C.c_char_p, _u32p, _u32p, _u32p, _u32p]),
+    "ctlproj_den_read": (_i, [C.c_char_p, _vp, _sz]),
+    "ctlproj_den_write": (_i, [C.c_char_p, _vp, _u32, _u32, _u32, _u32]),
+    "ctlproj_nrrd_read_header": (_i, [C.c_char_p, C.POINTER(NrrdInfo), C.c_char_p, _sz]),
+    "ctlproj_nrrd_read": (_i, [C.c_char_p, _vp, _sz]),
+    "ctlproj_nrrd_write": (_i, [C.c_char_p, _vp, _i, _u32, C.POINTER(C.c_uint64), C.c_char_p, _f64p, _f64p,
+                                C.c_char_p]),
     "ctlproj_diag_gather_probe": (_i, [_vp, _vp, _i, _vp, _sz, _f32p, _f64p]),
     "ctlproj_projector_autotune": (_i, [_vp, _vp, _f32p]),
     "ctlproj_autotune_table": (_i, [C.c_char_p, _sz, C.POINTER(_sz)]),

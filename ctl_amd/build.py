@@ -22,7 +22,8 @@ CLANG = str(ROCM / "llvm" / "bin" / "clang++")
 ARCH = "gfx950"
 
 DEVICE_SOURCES = ["kernels.hip"]
-HOST_SOURCES = ["api.cpp", "batch.cpp", "geometry.cpp", "phantom.cpp", "serve.cpp", "autotune.cpp"]
+HOST_SOURCES = ["api.cpp", "batch.cpp", "geometry.cpp", "phantom.cpp", "serve.cpp", "autotune.cpp",
+                "fileio.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 # Packed FP32 instructions are kept out of every device code object.

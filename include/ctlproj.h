@@ -37,7 +37,8 @@ typedef enum ctlproj_status {
     CTLPROJ_ERR_NO_DATA = 6,          /* volume without data (reference returns 0 views) */
     CTLPROJ_ERR_OVER_BUDGET = 7,      /* a request larger than the server's HBM budget */
     CTLPROJ_ERR_INJECTED = 8,         /* failure forced by ctlproj_debug_inject_failure */
-    CTLPROJ_ERR_TIMEOUT = 9           /* a wait ended before the request finished */
+    CTLPROJ_ERR_TIMEOUT = 9,          /* a wait ended before the request finished */
+    CTLPROJ_ERR_IO = 10               /* a file could not be read or written, or is malformed */
 } ctlproj_status;
 
 typedef enum ctlproj_dtype {
@@ -273,6 +274,55 @@ ctlproj_status ctlproj_phantom_ellipsoids(const uint32_t dims[3], const float vo
                                           float semi_axis_min_mm, float semi_axis_max_mm,
                                           float mu_min, float mu_max, float background_mu,
                                           float* out);
+
+/* ---- DEN and NRRD files (io/den/denfileio.h, io/nrrd/nrrdfileio.h) ------------------------
+ * DEN: 6-byte header of three uint16 rows, cols, count (den_header.tpp), then count matrices of
+ * rows x cols elements, cols fastest; the element size (1, 2, 4 or 8 bytes: uchar, ushort, float,
+ * double) follows from the file size. Volumes: cols = X, rows = Y, count = Z; projections:
+ * cols = channels, rows = detector rows, count = modules x views; geometry: 3 x 4 per (view,
+ * module). NRRD: "NRRD000<v>" header with type/dimension/sizes/encoding raw/endian little,
+ * optional spacings and space origin, key:=value pairs and comments, an empty line, raw data
+ * (nrrdfileio.tpp). All data little-endian; sizes of at most 4 dimensions, first fastest. */
+typedef enum ctlproj_elem {
+    CTLPROJ_ELEM_U8 = 1,
+    CTLPROJ_ELEM_U16 = 2,
+    CTLPROJ_ELEM_F32 = 4,
+    CTLPROJ_ELEM_F64 = 8
+} ctlproj_elem;
+
+typedef struct ctlproj_nrrd_header {
+    uint32_t version;       /* the digit of "NRRD000<v>" */
+    uint32_t ndim;          /* "dimension:" */
+    uint32_t nsizes;        /* number of "sizes:" entries (== ndim) */
+    int32_t elem;           /* ctlproj_elem of "type:" */
+    uint32_t elem_bytes;
+    uint64_t sizes[4];
+    uint64_t count;         /* product of the sizes */
+    uint64_t header_bytes;  /* offset of the raw data */
+    int32_t has_spacing;
+    double spacing[3];      /* "spacings:" (voxel size, mm) */
+    int32_t has_origin;
+    double origin[3];       /* "space origin:" (volume offset, mm) */
+    int32_t data_complete;  /* the file holds count x elem_bytes bytes of data */
+} ctlproj_nrrd_header;
+
+ctlproj_status ctlproj_den_info(const char* path, uint32_t* rows, uint32_t* cols, uint32_t* count,
+                                uint32_t* elem_bytes);
+/* Reads the data (out_bytes must equal rows*cols*count*elem_bytes). */
+ctlproj_status ctlproj_den_read(const char* path, void* out, size_t out_bytes);
+/* Sizes 1..65535 each (den_header.h isOutOfBounds). */
+ctlproj_status ctlproj_den_write(const char* path, const void* data, uint32_t elem_bytes,
+                                 uint32_t rows, uint32_t cols, uint32_t count);
+/* Header fields; keyvals (nullable) receives the key:=value lines, NUL-terminated. */
+ctlproj_status ctlproj_nrrd_read_header(const char* path, ctlproj_nrrd_header* info, char* keyvals,
+                                 size_t keyvals_cap);
+/* Reads the raw data (out_bytes must equal count x elem_bytes). */
+ctlproj_status ctlproj_nrrd_read(const char* path, void* out, size_t out_bytes);
+/* labels: the "labels:" value (nullable, e.g. "\"num vox x\" \"num vox y\" \"num vox z\"");
+ * spacing and origin: 3 values each or NULL; keyvals: "key:=value" lines or NULL. */
+ctlproj_status ctlproj_nrrd_write(const char* path, const void* data, int elem, uint32_t ndim,
+                                  const uint64_t* sizes, const char* labels, const double* spacing,
+                                  const double* origin, const char* keyvals);
 
 /* ---- gather-ceiling probes (diagnostics) -------------------------------------------------
  * No counterpart in the reference: they measure the ceiling against which the ray caster's gather

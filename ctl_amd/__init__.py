@@ -2,7 +2,8 @@
 from ._lib import CtlProjError, lib  # noqa: F401
 from .projector import (  # noqa: F401
     DEG, AcquisitionSetup, Batch, Server, autotune_table, AxialScanTrajectory, CarmGantry, CirclePlusLineTrajectory, Context,
-    CTSystem, CylindricalDetector, FlatPanelDetector, GeometryEncoder, HelicalTrajectory,
+    CTSystem, CylindricalDetector, DecodedGeometry, FlatPanelDetector, GeometryDecoder, GeometryEncoder,
+    HelicalTrajectory,
     ProjectionData, RayCasterProjector, ResidentVolume, Settings, ShortScanTrajectory,
     TubularGantry, VolumeData, VoxelVolume, WobbleTrajectory, ellipsoid_phantom,
 )

@@ -136,6 +136,9 @@ SIGNATURES: dict[str, tuple] = {
                                         C.c_float, C.c_float, C.c_float, _f32p]),
     "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
     "ctlproj_settings_optimized_for": (_i, [_f32p, C.c_double, C.c_double, C.POINTER(Settings)]),
+    "ctlproj_decode_geometry": (_i, [_f64p, _u32, _u32, _u32, _u32, _i, C.c_double, C.c_double, _f64p, _f64p,
+                                     _f64p, _f64p, _f64p, _f64p]),
+    "ctlproj_encode_generic": (_i, [_f64p, _f64p, _f64p, _f64p, _f64p, _u32, _u32, _u32, _u32, _f64p, _sz]),
     "ctlproj_den_info": (_i, [C.c_char_p, _u32p, _u32p, _u32p, _u32p]),
     "ctlproj_den_read": (_i, [C.c_char_p, _vp, _sz]),
     "ctlproj_den_write": (_i, [C.c_char_p, _vp, _u32, _u32, _u32, _u32]),

@@ -773,6 +773,69 @@ ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_vo
     return CTLPROJ_OK;
 }
 
+ctlproj_status ctlproj_decode_geometry(const double* pmats, uint32_t views, uint32_t modules,
+                                       uint32_t channels, uint32_t rows, int reference, double a,
+                                       double b, double* source_pos, double* source_rot,
+                                       double* pixel_size, double* skew, double* module_pos,
+                                       double* module_rot)
+{
+    if (!pmats || !source_pos || !source_rot || !pixel_size || !skew || !module_pos || !module_rot)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (modules == 0 || channels == 0 || rows == 0) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "empty detector");
+    if (reference < CTLPROJ_DECODE_PIXEL_SIZE || reference > CTLPROJ_DECODE_SOURCE_DETECTOR_DISTANCE)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "invalid value for the physical dimension");
+    if (!(a > 0.0) || (reference == CTLPROJ_DECODE_PIXEL_SIZE && !(b > 0.0)))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "reference lengths must be positive");
+    std::vector<ctlgeo::PMat> P(modules);
+    ctlgeo::DecodedView d;
+    for (uint32_t v = 0; v < views; ++v) {
+        for (uint32_t m = 0; m < modules; ++m)
+            std::memcpy(P[m].p, pmats + (size_t(v) * modules + m) * 12, 12 * sizeof(double));
+        if (!ctlgeo::decodeView(P.data(), modules, channels, rows, reference, a, b, &d))
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "singular projection matrix in view " + std::to_string(v));
+        std::memcpy(source_pos + 3 * size_t(v), d.source.v, 3 * sizeof(double));
+        std::memcpy(source_rot + 9 * size_t(v), d.source_rotation.m, 9 * sizeof(double));
+        pixel_size[2 * size_t(v)] = d.pixel_w;
+        pixel_size[2 * size_t(v) + 1] = d.pixel_h;
+        skew[v] = d.skew;
+        for (uint32_t m = 0; m < modules; ++m) {
+            const size_t k = size_t(v) * modules + m;
+            std::memcpy(module_pos + 3 * k, d.modules[m].position.v, 3 * sizeof(double));
+            std::memcpy(module_rot + 9 * k, d.modules[m].rotation.m, 9 * sizeof(double));
+        }
+    }
+    return CTLPROJ_OK;
+}
+
+ctlproj_status ctlproj_encode_generic(const double* source_pos, const double* module_pos,
+                                      const double* module_rot, const double* pixel_size,
+                                      const double* skew, uint32_t views, uint32_t modules,
+                                      uint32_t channels, uint32_t rows, double* out, size_t out_count)
+{
+    if (!source_pos || !module_pos || !module_rot || !pixel_size || !out)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (out_count != size_t(views) * modules * 12)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output must hold views*modules*12 values");
+    if (modules == 0 || channels == 0 || rows == 0) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "empty detector");
+    std::vector<ctlgeo::Location> locs(modules);
+    std::vector<ctlgeo::PMat> P(modules);
+    for (uint32_t v = 0; v < views; ++v) {
+        ctlgeo::Vec3 src;
+        std::memcpy(src.v, source_pos + 3 * size_t(v), 3 * sizeof(double));
+        for (uint32_t m = 0; m < modules; ++m) {
+            const size_t k = size_t(v) * modules + m;
+            std::memcpy(locs[m].position.v, module_pos + 3 * k, 3 * sizeof(double));
+            std::memcpy(locs[m].rotation.m, module_rot + 9 * k, 9 * sizeof(double));
+        }
+        const double pw = pixel_size[2 * size_t(v)], ph = pixel_size[2 * size_t(v) + 1];
+        if (!(pw > 0.0) || !(ph > 0.0)) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "pixel size must be positive");
+        ctlgeo::encodeModules(src, locs.data(), modules, channels, rows, pw, ph, skew ? skew[v] : 0.0, P.data());
+        for (uint32_t m = 0; m < modules; ++m)
+            std::memcpy(out + (size_t(v) * modules + m) * 12, P[m].p, 12 * sizeof(double));
+    }
+    return CTLPROJ_OK;
+}
+
 // ---- gather-ceiling probes (diagnostics) -----------------------------------------------------
 
 ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_volume* vol, int mode,

@@ -160,20 +160,14 @@ void normalize(PMat& P)
     for (double& x : P.p) x /= denom;
 }
 
-void encodeSingleView(const Gantry& g, const Detector& d, const Vec3& focal_spot, PMat* out)
+void encodeModules(const Vec3& src, const Location* modules, size_t n, uint32_t channels, uint32_t rows,
+                   double pw, double ph, double skew, PMat* out)
 {
-    const Vec3 src = g.sourcePosition() + sourceRotation(g) * focal_spot; // geometryencoder.cpp:289-293
-    const Vec3 det_pos = g.detectorPosition();
-    const Mat3 det_rot = g.detectorRotation();
-    const Mat3 det_rot_t = det_rot.transposed();
-    const auto locs = d.moduleLocations();
-    for (size_t m = 0; m < locs.size(); ++m) {
-        const Vec3 mod_pos = det_pos + det_rot_t * locs[m].position;
-        const Mat3 total_rot = locs[m].rotation * det_rot;
-        const Vec3 dev = total_rot * (mod_pos - src);
-        const Mat3 K = intrinsic(dev, d.channels, d.rows, d.pixel_w, d.pixel_h, d.skew);
+    for (size_t m = 0; m < n; ++m) {
+        const Vec3 dev = modules[m].rotation * (modules[m].position - src);
+        const Mat3 K = intrinsic(dev, channels, rows, pw, ph, skew);
         // ProjectionMatrix::compose(K, R, c) = [K R | -K R c] (projectionmatrix.cpp:43-48)
-        const Mat3 KR = K * total_rot;
+        const Mat3 KR = K * modules[m].rotation;
         const Vec3 t = KR * src;
         PMat P;
         for (int r = 0; r < 3; ++r) {
@@ -183,6 +177,109 @@ void encodeSingleView(const Gantry& g, const Detector& d, const Vec3& focal_spot
         normalize(P);
         out[m] = P;
     }
+}
+
+namespace {
+// M = R Q with R upper triangular, positive diagonal, Q orthogonal: Gram-Schmidt on the rows of M
+// from the last one up. This is the unique RQ decomposition (mat::RQdecomposition with `unique`,
+// matrix_algorithm.cpp:67-87); it needs a regular M.
+bool rq(const Mat3& M, Mat3& R, Mat3& Q)
+{
+    R = Mat3{ { 0, 0, 0, 0, 0, 0, 0, 0, 0 } };
+    for (int i = 2; i >= 0; --i) {
+        double v[3] = { M(i, 0), M(i, 1), M(i, 2) };
+        for (int j = i + 1; j < 3; ++j) {
+            const double rij = v[0] * Q(j, 0) + v[1] * Q(j, 1) + v[2] * Q(j, 2);
+            R(i, j) = rij;
+            for (int k = 0; k < 3; ++k) v[k] -= rij * Q(j, k);
+        }
+        const double nrm = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (!(nrm > 0.0)) return false;
+        R(i, i) = nrm;
+        for (int k = 0; k < 3; ++k) Q(i, k) = v[k] / nrm;
+    }
+    return true;
+}
+
+Vec3 cross(const Vec3& a, const Vec3& b)
+{
+    return Vec3{ { a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0] } };
+}
+} // namespace
+
+bool decodeView(const PMat* P, uint32_t n, uint32_t channels, uint32_t rows, int reference, double a, double b,
+                DecodedView* out)
+{
+    // source position from module 0 (ProjectionMatrix::sourcePosition, projectionmatrix.cpp:264-274)
+    const double* p = P[0].p;
+    auto det = [&](int c0, int c1, int c2) {
+        const Mat3 m{ { p[c0], p[c1], p[c2], p[4 + c0], p[4 + c1], p[4 + c2], p[8 + c0], p[8 + c1], p[8 + c2] } };
+        return det3(m);
+    };
+    const double dm = det(0, 1, 2);
+    if (dm == 0.0) return false;
+    out->source = Vec3{ { det(1, 2, 3) / -dm, -det(0, 2, 3) / -dm, det(0, 1, 3) / -dm } };
+    // intrinsics of module 0 (intrinsicMatK: R of the unique RQ scaled to R22 = 1): focal lengths
+    // in pixels and skew (geometrydecoder.cpp computePixelSize)
+    Mat3 R, Q;
+    const Mat3 M0{ { p[0], p[1], p[2], p[4], p[5], p[6], p[8], p[9], p[10] } };
+    if (!rq(M0, R, Q)) return false;
+    const double fx = R(0, 0) / R(2, 2), fy = R(1, 1) / R(2, 2);
+    out->skew = R(0, 1) / R(2, 2);
+    switch (reference) {
+    case DecodePixelSize: out->pixel_w = a; out->pixel_h = b; break;
+    case DecodePixelWidth: out->pixel_w = a; out->pixel_h = a * fx / fy; break;
+    case DecodePixelHeight: out->pixel_w = a * fy / fx; out->pixel_h = a; break;
+    case DecodeSourceDetectorDistance: out->pixel_w = a / fx; out->pixel_h = a / fy; break;
+    default: return false;
+    }
+    // module locations (computeModuleLocations): centre pixel direction M^-1 [x, y, 1] scaled by
+    // R00 (directionSourceToPixel, NormalizeByX) and the pixel width, from the source; rotation Q
+    out->modules.resize(n);
+    const double cx = 0.5 * double(channels - 1), cy = 0.5 * double(rows - 1);
+    Vec3 centroid{ { 0, 0, 0 } };
+    for (uint32_t m = 0; m < n; ++m) {
+        const double* q = P[m].p;
+        const Mat3 Mm{ { q[0], q[1], q[2], q[4], q[5], q[6], q[8], q[9], q[10] } };
+        if (!rq(Mm, R, Q)) return false;
+        const double sgn = det3(Mm) < 0.0 ? -1.0 : 1.0; // sign(det M) M^-1 [x, y, 1]
+        const double dz = 1.0 / R(2, 2);
+        const double dy = (cy - dz * R(1, 2)) / R(1, 1);
+        const double dx = (cx - dy * R(0, 1) - dz * R(0, 2)) / R(0, 0);
+        const Vec3 d = Q.transposed() * Vec3{ { dx, dy, dz } };
+        const double s = sgn * R(0, 0) * out->pixel_w;
+        out->modules[m].position = out->source + Vec3{ { d[0] * s, d[1] * s, d[2] * s } };
+        out->modules[m].rotation = Q;
+        centroid = centroid + out->modules[m].position;
+    }
+    // source rotation (computeSourceRotation): columns r1, r2, s with s the unit vector from the
+    // source to the module centroid, r1 = s x e (e the axis least aligned with s), r2 = s x r1
+    for (int k = 0; k < 3; ++k) centroid[k] /= double(n);
+    Vec3 s = centroid - out->source;
+    const double ns = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    if (!(ns > 0.0)) return false;
+    for (int k = 0; k < 3; ++k) s[k] /= ns;
+    int axis = std::fabs(s[0]) < std::fabs(s[1]) ? 0 : 1;
+    axis = std::fabs(s[axis]) < std::fabs(s[2]) ? axis : 2;
+    Vec3 e{ { 0, 0, 0 } };
+    e[axis] = 1.0;
+    const Vec3 r1 = cross(s, e), r2 = cross(s, r1);
+    out->source_rotation = Mat3{ { r1[0], r2[0], s[0], r1[1], r2[1], s[1], r1[2], r2[2], s[2] } };
+    return true;
+}
+
+void encodeSingleView(const Gantry& g, const Detector& d, const Vec3& focal_spot, PMat* out)
+{
+    const Vec3 src = g.sourcePosition() + sourceRotation(g) * focal_spot; // geometryencoder.cpp:289-293
+    const Vec3 det_pos = g.detectorPosition();
+    const Mat3 det_rot = g.detectorRotation();
+    const Mat3 det_rot_t = det_rot.transposed();
+    auto locs = d.moduleLocations();
+    for (auto& l : locs) { // module locations in world coordinates
+        l.position = det_pos + det_rot_t * l.position;
+        l.rotation = l.rotation * det_rot;
+    }
+    encodeModules(src, locs.data(), locs.size(), d.channels, d.rows, d.pixel_w, d.pixel_h, d.skew, out);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -6,6 +6,7 @@
 // reference's to rounding.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -93,5 +94,28 @@ bool prepareView(const Trajectory& t, uint32_t view, uint32_t n_views, const Det
 
 // Normalisation of a P-matrix (ProjectionMatrix::normalized, projectionmatrix.cpp:143-150).
 void normalize(PMat& P);
+
+// Encoding of generic components (GeometryEncoder::encodeSingleViewGeometry for a GenericGantry /
+// GenericDetector): source position and per-module world locations (position of the module
+// centre, rotation world -> module), pixels per module and pixel size.
+void encodeModules(const Vec3& src, const Location* modules, size_t n, uint32_t channels, uint32_t rows,
+                   double pw, double ph, double skew, PMat* out);
+
+// GeometryDecoder (geometrydecoder.cpp): physical geometry of one view from its P-matrices.
+enum DecodeReference {
+    DecodePixelSize = 0,              // pixel width and height given
+    DecodePixelWidth = 1,             // PhysicalDimension::PixelWidth
+    DecodePixelHeight = 2,            // PhysicalDimension::PixelHeight
+    DecodeSourceDetectorDistance = 3, // PhysicalDimension::SourceDetectorDistance
+};
+struct DecodedView {
+    Vec3 source;
+    Mat3 source_rotation; // computeSourceRotation (columns r1, r2, source->detector direction)
+    double pixel_w = 0, pixel_h = 0, skew = 0;
+    std::vector<Location> modules; // module centres and rotations (world -> module)
+};
+// reference DecodePixelSize: a, b = pixel width, height; otherwise a = the reference length in mm.
+bool decodeView(const PMat* P, uint32_t n_modules, uint32_t channels, uint32_t rows, int reference,
+                double a, double b, DecodedView* out);
 
 } // namespace ctlgeo

@@ -229,6 +229,63 @@ class GeometryEncoder:
         return out
 
 
+class DecodedGeometry:
+    """Physical geometry behind P-matrices, as GeometryDecoder builds it for a GenericGantry and
+    GenericDetector: per view the source location, pixel size and skew, per view and module the
+    module location (centre position, rotation world -> module)."""
+
+    def __init__(self, channels, rows, source_positions, source_rotations, pixel_sizes, skew,
+                 module_positions, module_rotations):
+        self.channels, self.rows = channels, rows
+        self.source_positions = source_positions    # (V, 3)
+        self.source_rotations = source_rotations    # (V, 3, 3), columns r1, r2, source->detector
+        self.pixel_sizes = pixel_sizes              # (V, 2) width, height
+        self.skew = skew                            # (V,)
+        self.module_positions = module_positions    # (V, M, 3)
+        self.module_rotations = module_rotations    # (V, M, 3, 3)
+
+    def nbViews(self) -> int:
+        return self.source_positions.shape[0]
+
+    def encode(self) -> np.ndarray:
+        """GeometryEncoder::encodeFullGeometry of the decoded setup: (V, M, 3, 4)."""
+        V, M = self.module_positions.shape[:2]
+        out = np.zeros((V, M, 3, 4), np.float64)
+        f = lambda a: _lib.dptr(np.ascontiguousarray(a, np.float64))
+        check(lib().ctlproj_encode_generic(f(self.source_positions), f(self.module_positions),
+                                           f(self.module_rotations), f(self.pixel_sizes), f(self.skew),
+                                           V, M, self.channels, self.rows, _lib.dptr(out), out.size))
+        return out
+
+
+class GeometryDecoder:
+    """GeometryDecoder (geometrydecoder.h:25-74): decodes P-matrices into the physical geometry,
+    given the pixels per module and either the pixel size or one physical length."""
+    PixelWidth, PixelHeight, SourceDetectorDistance = 1, 2, 3  # PhysicalDimension
+
+    def __init__(self, pixelPerModule, pixelDimensions=None, physicalDimension=None, mm=None):
+        self.pixelPerModule = tuple(int(v) for v in pixelPerModule)
+        if pixelDimensions is not None:
+            self._ref, self._a, self._b = 0, float(pixelDimensions[0]), float(pixelDimensions[1])
+        elif physicalDimension is not None and mm is not None:
+            self._ref, self._a, self._b = int(physicalDimension), float(mm), 0.0
+        else:
+            raise ValueError("give the pixel dimensions or a physical dimension with its length")
+
+    def decodeFullGeometry(self, pmats) -> DecodedGeometry:
+        P = np.ascontiguousarray(pmats, np.float64)
+        if P.ndim == 3:
+            P = P[:, None]
+        V, M = P.shape[:2]
+        C_, R_ = self.pixelPerModule
+        src = np.zeros((V, 3)); srot = np.zeros((V, 3, 3)); px = np.zeros((V, 2)); sk = np.zeros(V)
+        mpos = np.zeros((V, M, 3)); mrot = np.zeros((V, M, 3, 3))
+        d = _lib.dptr
+        check(lib().ctlproj_decode_geometry(d(P), V, M, C_, R_, self._ref, self._a, self._b, d(src), d(srot),
+                                            d(px), d(sk), d(mpos), d(mrot)))
+        return DecodedGeometry(C_, R_, src, srot, px, sk, mpos, mrot)
+
+
 # ----------------------------------------------------------------------------------------------
 # environment and projector
 

@@ -265,6 +265,34 @@ ctlproj_status ctlproj_encode_carm_poses(double carm_span, const ctlproj_detecto
                                          const double* positions, const double* rotations,
                                          uint32_t views, double* out, size_t out_count);
 
+/* GeometryDecoder::decodeFullGeometry (geometrydecoder.cpp): the physical geometry behind
+ * P-matrices [V][M][3][4] of modules with channels x rows pixels. reference: CTLPROJ_DECODE_PIXEL_SIZE
+ * (a, b = pixel width, height in mm) or the PhysicalDimension whose length in mm is `a`
+ * (pixel width, pixel height, source-detector distance). Outputs per view: source position (3),
+ * source rotation (9, row-major; computeSourceRotation's columns r1, r2, unit source-to-detector
+ * direction), pixel size (2), skew (1); per view and module: centre position (3) and rotation
+ * (9, row-major, world -> module). */
+typedef enum ctlproj_decode_reference {
+    CTLPROJ_DECODE_PIXEL_SIZE = 0,
+    CTLPROJ_DECODE_PIXEL_WIDTH = 1,
+    CTLPROJ_DECODE_PIXEL_HEIGHT = 2,
+    CTLPROJ_DECODE_SOURCE_DETECTOR_DISTANCE = 3
+} ctlproj_decode_reference;
+ctlproj_status ctlproj_decode_geometry(const double* pmats, uint32_t views, uint32_t modules,
+                                       uint32_t channels, uint32_t rows, int reference, double a,
+                                       double b, double* source_pos, double* source_rot,
+                                       double* pixel_size, double* skew, double* module_pos,
+                                       double* module_rot);
+
+/* GeometryEncoder for generic components: P-matrices [V][M][3][4] (normalised) from source
+ * positions (V x 3), module centres (V x M x 3), module rotations (V x M x 9, world -> module),
+ * pixel sizes (V x 2) and skew coefficients (V, nullable = 0). The inverse of
+ * ctlproj_decode_geometry. */
+ctlproj_status ctlproj_encode_generic(const double* source_pos, const double* module_pos,
+                                      const double* module_rot, const double* pixel_size,
+                                      const double* skew, uint32_t views, uint32_t modules,
+                                      uint32_t channels, uint32_t rows, double* out, size_t out_count);
+
 /* ---- synthetic phantoms ------------------------------------------------------------------ */
 
 /* Seeded random-ellipsoid phantom (host memory, x fastest), see the generator description in

@@ -73,3 +73,69 @@ def test_carm_circle_source_positions_and_projection():
     ang = [np.arctan2(*oracle.source_position(P[v])[[1, 0]]) for v in range(36)]
     steps = np.diff(np.unwrap(ang))
     assert np.allclose(steps, 10.0 * DEG, atol=1e-12)
+
+
+# ---- GeometryDecoder (geometrydecoder.cpp) --------------------------------------------------
+
+def _max_rel(A, B):
+    A, B = normalized(A), normalized(B)
+    return float(np.max(np.linalg.norm((A - B).reshape(len(A), -1, 12), axis=-1) /
+                        np.linalg.norm(B.reshape(len(B), -1, 12), axis=-1)))
+
+
+@pytest.mark.parametrize("name,views", [("tubeGeo.den", 10), ("cArmGeo.den", 30)])
+def test_decoder_round_trip_on_reference_fixtures(golden, name, views):
+    """Decoding the reference's matrices (16x64-pixel modules of 1x1 mm) and encoding the decoded
+    generic setup again gives the same matrices."""
+    from ctl_amd import GeometryDecoder
+    P = read_den_pmats(golden / name, 40)
+    dec = GeometryDecoder((16, 64), (1.0, 1.0)).decodeFullGeometry(P)
+    assert dec.module_positions.shape == (views, 40, 3)
+    assert _max_rel(dec.encode(), P) <= 1e-9
+    # rotations are orthonormal with determinant +1 (normalised P: det M > 0)
+    R = dec.module_rotations.reshape(-1, 3, 3)
+    assert np.allclose(R @ R.transpose(0, 2, 1), np.eye(3), atol=1e-12)
+    assert np.allclose(np.linalg.det(R), 1.0, atol=1e-12)
+    # the pixel-width reference recovers the square 1 mm pixels from the focal lengths
+    dec_w = GeometryDecoder((16, 64), physicalDimension=GeometryDecoder.PixelWidth, mm=1.0).decodeFullGeometry(P)
+    assert np.allclose(dec_w.pixel_sizes, 1.0, rtol=1e-9)
+    assert np.allclose(dec_w.module_positions, dec.module_positions, rtol=0, atol=1e-9)
+
+
+def test_decoder_carm_poses_closed_form():
+    """C-arm at known poses (carmgantry.cpp:48-71): the source is the pose position, the flat panel
+    centre lies span mm along the pose's z axis, its rotation is the pose rotation transposed, and
+    the source-detector distance reference gives back the pixel size."""
+    from ctl_amd import GeometryDecoder
+    rng = np.random.default_rng(3)
+    V, span, pix = 5, 1200.0, (0.8, 0.6)
+    pos = rng.normal(0.0, 300.0, (V, 3))
+    rot = np.stack([np.linalg.qr(rng.normal(size=(3, 3)))[0] for _ in range(V)])
+    rot[np.linalg.det(rot) < 0] *= -1.0
+    setup = AcquisitionSetup(CTSystem(CarmGantry(span), FlatPanelDetector((256, 128), pix)), V)
+    setup.setCarmPoses(pos, rot.reshape(V, 9))
+    P = GeometryEncoder.encodeFullGeometry(setup)
+    dec = GeometryDecoder((256, 128), pix).decodeFullGeometry(P)
+    assert np.allclose(dec.source_positions, pos, rtol=0, atol=1e-9)
+    assert np.allclose(dec.module_positions[:, 0], pos + rot @ np.array([0.0, 0.0, span]), rtol=0, atol=1e-8)
+    assert np.allclose(dec.module_rotations[:, 0], rot.transpose(0, 2, 1), atol=1e-12)
+    assert np.allclose(dec.skew, 0.0, atol=1e-12)
+    dec_sdd = GeometryDecoder((256, 128), physicalDimension=GeometryDecoder.SourceDetectorDistance,
+                              mm=span).decodeFullGeometry(P)
+    assert np.allclose(dec_sdd.pixel_sizes, pix, rtol=1e-12)
+    # computeSourceRotation: third column the unit vector to the detector centre, r1 and r2 normal to it
+    s = dec.source_rotations[:, :, 2]
+    to_det = dec.module_positions[:, 0] - pos
+    assert np.allclose(s, to_det / np.linalg.norm(to_det, axis=1)[:, None], atol=1e-12)
+    assert np.allclose(np.einsum("vi,vi->v", dec.source_rotations[:, :, 0], s), 0.0, atol=1e-12)
+    assert np.allclose(np.einsum("vi,vi->v", dec.source_rotations[:, :, 1], s), 0.0, atol=1e-12)
+    assert _max_rel(dec.encode(), P) <= 1e-12
+
+
+def test_decoder_rejects_bad_references():
+    from ctl_amd import CtlProjError, GeometryDecoder
+    P = np.zeros((1, 1, 3, 4))
+    with pytest.raises(CtlProjError):  # singular matrix
+        GeometryDecoder((4, 4), (1.0, 1.0)).decodeFullGeometry(P)
+    with pytest.raises(ValueError):
+        GeometryDecoder((4, 4))

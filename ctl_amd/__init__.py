@@ -1,7 +1,7 @@
 """MI355X-native forward projector with CTL's RayCasterProjector interface."""
 from ._lib import CtlProjError, lib  # noqa: F401
 from .projector import (  # noqa: F401
-    DEG, AcquisitionSetup, Batch, Server, autotune_table, AxialScanTrajectory, CarmGantry, CirclePlusLineTrajectory, Context,
+    DEG, AcquisitionSetup, ArealFocalSpotExtension, Batch, Server, autotune_table, AxialScanTrajectory, CarmGantry, CirclePlusLineTrajectory, Context,
     CTSystem, CylindricalDetector, DecodedGeometry, FlatPanelDetector, GeometryDecoder, GeometryEncoder,
     HelicalTrajectory,
     ProjectionData, RayCasterProjector, ResidentVolume, Settings, ShortScanTrajectory,

@@ -136,6 +136,13 @@ SIGNATURES: dict[str, tuple] = {
                                         C.c_float, C.c_float, C.c_float, _f32p]),
     "ctlproj_debug_inject_failure": (_i, [_vp, _i]),
     "ctlproj_settings_optimized_for": (_i, [_f32p, C.c_double, C.c_double, C.POINTER(Settings)]),
+    "ctlproj_encode_trajectory_shifted": (_i, [C.POINTER(Gantry), C.POINTER(Detector), C.POINTER(Trajectory),
+                                               _u32, _f64p, _f64p, _sz]),
+    "ctlproj_encode_carm_poses_shifted": (_i, [C.c_double, C.POINTER(Detector), _f64p, _f64p, _u32, _f64p, _f64p,
+                                               _sz]),
+    "ctlproj_project_composite": (_i, [_vp, C.POINTER(_vp), _u32, _f32p, _sz]),
+    "ctlproj_project_composite_device": (_i, [_vp, C.POINTER(_vp), _u32, _vp, _sz]),
+    "ctlproj_average_projections": (_i, [_vp, _vp, _u32, _sz, _i, _vp]),
     "ctlproj_decode_geometry": (_i, [_f64p, _u32, _u32, _u32, _u32, _i, C.c_double, C.c_double, _f64p, _f64p,
                                      _f64p, _f64p, _f64p, _f64p]),
     "ctlproj_encode_generic": (_i, [_f64p, _f64p, _f64p, _f64p, _f64p, _u32, _u32, _u32, _u32, _f64p, _sz]),

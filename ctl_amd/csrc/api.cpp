@@ -309,12 +309,12 @@ ctlproj_status checkProject(ctlproj_projector* p, const ctlproj_volume* vol, con
 }
 
 ctlproj_status launchCast(ctlproj_projector* p, const EffectiveVolume& ev, uint32_t v0, uint32_t nv,
-                          float* d_out)
+                          float* d_out, int accumulate = 0)
 {
     const ctlk::RayParams rp = rayParams(p, nv);
     const ctlk::LaunchShape sh = shapeFor(p->s, ev.view.storage, ev.view.layout, p->C, p->R);
     hipError_t e = ctlk::launchRaycast(ev.view, ev.geom, p->d_geom + size_t(v0) * p->M, rp, sh,
-                                       d_out, p->stream);
+                                       d_out, p->stream, accumulate);
     return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "raycast kernel");
 }
 
@@ -596,6 +596,60 @@ ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume
         if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return hipFail(e, "hipStreamSynchronize");
     }
     return CTLPROJ_OK;
+}
+
+ctlproj_status ctlproj_project_composite_device(ctlproj_projector* p, const ctlproj_volume* const* vols,
+                                                uint32_t n, float* device_out, size_t out_count)
+{
+    if (!vols || n == 0) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "no volumes");
+    ctlproj_status st;
+    for (uint32_t k = 0; k < n; ++k)
+        if ((st = checkProject(p, vols[k], device_out, out_count)) != CTLPROJ_OK) return st;
+    if (p->V == 0) return CTLPROJ_OK;
+    if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+    hipError_t e = hipEventRecord(p->ev_start, p->stream);
+    if (e != hipSuccess) return hipFail(e, "hipEventRecord");
+    // one stream: the sub-volumes' projections add up in launch order, deterministically
+    for (uint32_t k = 0; k < n; ++k) {
+        EffectiveVolume ev;
+        if ((st = effectiveVolume(vols[k], p->s, p->stream, ev)) != CTLPROJ_OK) return st;
+        if ((st = prepareGeometry(p, ev)) != CTLPROJ_OK) return st;
+        if ((st = launchCast(p, ev, 0, p->V, device_out, k > 0 ? 1 : 0)) != CTLPROJ_OK) return st;
+        if (ev.tmp && (e = hipStreamSynchronize(p->stream)) != hipSuccess) // temporary outlives the kernel
+            return hipFail(e, "hipStreamSynchronize");
+    }
+    if ((e = hipEventRecord(p->ev_stop, p->stream)) != hipSuccess) return hipFail(e, "hipEventRecord");
+    p->timed = true;
+    return CTLPROJ_OK;
+}
+
+ctlproj_status ctlproj_project_composite(ctlproj_projector* p, const ctlproj_volume* const* vols, uint32_t n,
+                                         float* out, size_t out_count)
+{
+    if (!p) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "projector is NULL");
+    if (!out && out_count) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output pointer is NULL");
+    ctlproj_status st = setDevice(p->ctx);
+    if (st != CTLPROJ_OK) return st;
+    const size_t need = size_t(p->V) * p->M * p->R * p->C;
+    if (need && out_count == need && (st = ensureDevice(p->d_out, p->out_cap, need)) != CTLPROJ_OK) return st;
+    if ((st = ctlproj_project_composite_device(p, vols, n, p->d_out, out_count)) != CTLPROJ_OK) return st;
+    if (need == 0) return CTLPROJ_OK;
+    hipError_t e = hipMemcpyAsync(out, p->d_out, need * sizeof(float), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "D2H copy");
+}
+
+ctlproj_status ctlproj_average_projections(ctlproj_context* ctx, const float* device_in, uint32_t k,
+                                           size_t n, int intensity_domain, float* device_out)
+{
+    if (!ctx || (k && n && (!device_in || !device_out))) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (k == 0) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "no projections to average");
+    ctlproj_status st = setDevice(ctx);
+    if (st != CTLPROJ_OK) return st;
+    if (injected(ctx)) return CTLPROJ_ERR_INJECTED;
+    hipError_t e = ctlk::launchAverage(device_in, k, n, intensity_domain ? 1 : 0, device_out, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "average kernel");
 }
 
 ctlproj_status ctlproj_projector_synchronize(ctlproj_projector* p)
@@ -915,6 +969,13 @@ ctlproj_status ctlproj_encode_trajectory(const ctlproj_gantry* gantry, const ctl
                                          const ctlproj_trajectory* traj, uint32_t views,
                                          double* out, size_t out_count)
 {
+    return ctlproj_encode_trajectory_shifted(gantry, det, traj, views, nullptr, out, out_count);
+}
+
+ctlproj_status ctlproj_encode_trajectory_shifted(const ctlproj_gantry* gantry, const ctlproj_detector* det,
+                                                 const ctlproj_trajectory* traj, uint32_t views,
+                                                 const double source_shift[3], double* out, size_t out_count)
+{
     ctlgeo::Detector d;
     if (!gantry || !traj || !toDetector(det, d)) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "bad system description");
     if (out_count != size_t(views) * d.modules * 12 || (views && !out))
@@ -932,7 +993,9 @@ ctlproj_status ctlproj_encode_trajectory(const ctlproj_gantry* gantry, const ctl
     t.type = ctlgeo::Trajectory::Type(traj->type);
     for (int i = 0; i < 6; ++i) t.p[i] = traj->p[i];
     std::vector<ctlgeo::PMat> pm(d.modules);
-    const ctlgeo::Vec3 focal{ { 0, 0, 0 } };
+    ctlgeo::Vec3 focal{ { 0, 0, 0 } };
+    if (source_shift)
+        for (int k = 0; k < 3; ++k) focal[k] = source_shift[k];
     for (uint32_t v = 0; v < views; ++v) {
         if (!ctlgeo::prepareView(t, v, views, d, g))
             return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "trajectory is not applicable to the gantry type");
@@ -947,6 +1010,15 @@ ctlproj_status ctlproj_encode_carm_poses(double carm_span, const ctlproj_detecto
                                          const double* positions, const double* rotations,
                                          uint32_t views, double* out, size_t out_count)
 {
+    return ctlproj_encode_carm_poses_shifted(carm_span, det, positions, rotations, views, nullptr, out,
+                                             out_count);
+}
+
+ctlproj_status ctlproj_encode_carm_poses_shifted(double carm_span, const ctlproj_detector* det,
+                                                 const double* positions, const double* rotations,
+                                                 uint32_t views, const double source_shift[3], double* out,
+                                                 size_t out_count)
+{
     ctlgeo::Detector d;
     if (!toDetector(det, d)) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "bad detector description");
     if (views && (!positions || !rotations || !out))
@@ -957,7 +1029,9 @@ ctlproj_status ctlproj_encode_carm_poses(double carm_span, const ctlproj_detecto
     g.type = ctlgeo::Gantry::CArm;
     g.carm_span = carm_span;
     std::vector<ctlgeo::PMat> pm(d.modules);
-    const ctlgeo::Vec3 focal{ { 0, 0, 0 } };
+    ctlgeo::Vec3 focal{ { 0, 0, 0 } };
+    if (source_shift)
+        for (int k = 0; k < 3; ++k) focal[k] = source_shift[k];
     for (uint32_t v = 0; v < views; ++v) {
         for (int k = 0; k < 3; ++k) g.carm_location.position[k] = positions[3 * v + k];
         for (int k = 0; k < 9; ++k) g.carm_location.rotation.m[k] = rotations[9 * v + k];

@@ -124,6 +124,7 @@ struct CastConst {
     Pitch pitch;
     uint32_t chunk;      // LDS-staged kernel: samples per ray and chunk (0: unstaged kernel)
     uint32_t lds_texels; // LDS-staged kernel: capacity of the LDS box in 8-byte texels
+    uint32_t accumulate; // 1: add to the output instead of overwriting it
 };
 
 CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
@@ -162,10 +163,11 @@ hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, 
                        uint32_t rows, const VolumeGeom& vg, int interpolate, RayGeom* d_geom,
                        hipStream_t s);
 
-// Ray casting of all views in one launch; out is [views][modules][rows][channels].
+// Ray casting of all views in one launch; out is [views][modules][rows][channels]. accumulate != 0
+// adds the projection to what d_out holds.
 hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                          const RayParams& rp, const LaunchShape& shape, float* d_out,
-                         hipStream_t s);
+                         hipStream_t s, int accumulate = 0);
 
 // Reference loop-count of all rays, added into *d_count.
 hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
@@ -192,6 +194,9 @@ hipError_t launchZero(float* d, size_t n, hipStream_t s);
 hipError_t launchProbeGather(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                              const RayParams& rp, float* d_out, hipStream_t s);
 hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, float* d_out, hipStream_t s);
+
+// Mean of k contiguous projections of n values (intensity != 0: -ln of the mean of exp(-value)).
+hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity, float* d_out, hipStream_t s);
 
 bool validShape(const LaunchShape& s);
 

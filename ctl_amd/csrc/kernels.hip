@@ -12,6 +12,7 @@
 // touching memory, so no index can fault even if a position were computed wrongly.
 #include "kernels.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <cstdint>
@@ -435,9 +436,11 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                 acc += sum;
             }
     }
-    // raycasterprojector_interp.cl:70-71
-    if (pixel)
-        out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+    if (pixel) { // raycasterprojector_interp.cl:70-71; accumulating launches add to the output
+        const float r = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+        float& o = out[(size_t(vm) * rp.rows + y) * rp.channels + x];
+        o = kc.accumulate ? o + r : r;
+    }
 }
 
 template <Storage ST, Layout LY, bool INTERP, int UNROLL>
@@ -716,8 +719,11 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             acc += chunk_sum;
         }
     }
-    if (pixel)
-        out[(size_t(vm) * rp.rows + y) * rp.channels + x] = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+    if (pixel) { // raycasterprojector_interp.cl:70-71; accumulating launches add to the output
+        const float r = kc.inc * acc / float(rp.rpp_x * rp.rpp_y);
+        float& o = out[(size_t(vm) * rp.rows + y) * rp.channels + x];
+        o = kc.accumulate ? o + r : r;
+    }
 }
 
 template <Storage ST, int UNROLL>
@@ -856,6 +862,26 @@ __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict
     if (mine) atomicAdd(&block_sum, mine);
     __syncthreads();
     if (threadIdx.x == 0 && block_sum) atomicAdd(count, block_sum);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mean of k projections of n values each (contiguous), in the extinction domain or, for
+// intensity-domain averaging, as -ln(mean(exp(-e))) (ArealFocalSpotExtension,
+// arealfocalspotextension.cpp:82-166, with initial intensity 1).
+
+__global__ __launch_bounds__(kBlock) void average_kernel(const float* __restrict__ in, uint32_t k, size_t n,
+                                                         int intensity, float* __restrict__ out)
+{
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float s = 0.0f;
+        for (uint32_t j = 0; j < k; ++j) {
+            const float e = in[size_t(j) * n + i];
+            s += intensity ? expf(-e) : e;
+        }
+        s /= float(k);
+        out[i] = intensity ? -logf(s) : s;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1187,6 +1213,14 @@ hipError_t launchProbeGather(const VolumeView& vol, const VolumeGeom& vg, const 
     return hipGetLastError();
 }
 
+hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity, float* d_out, hipStream_t s)
+{
+    if (k == 0 || n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 65536);
+    average_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_in, k, n, intensity, d_out);
+    return hipGetLastError();
+}
+
 hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, float* d_out, hipStream_t s)
 {
     if (grid == 0 || iters % 4) return hipErrorInvalidValue;
@@ -1215,10 +1249,11 @@ hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, 
 
 hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                          const RayParams& rp, const LaunchShape& shape, float* d_out,
-                         hipStream_t s)
+                         hipStream_t s, int accumulate)
 {
     if (!validShape(shape)) return hipErrorInvalidValue;
-    const CastConst kc = makeCastConst(&vol, vg, rp, shape);
+    CastConst kc = makeCastConst(&vol, vg, rp, shape);
+    kc.accumulate = accumulate ? 1u : 0u;
     const uint64_t blocks = uint64_t(kc.tiles_per_vm) * rp.views * rp.modules;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;

@@ -142,6 +142,7 @@ def _gantry(g) -> _lib.Gantry:
 class CTSystem:
     gantry: object
     detector: object
+    focalSpotSize: tuple = (0.0, 0.0)  # mm (width, height) of the X-ray source's focal spot
 
 
 def _traj(kind: int, *p: float) -> _lib.Trajectory:
@@ -207,25 +208,35 @@ class AcquisitionSetup:
         self._protocol = None
 
 
+def _module_pixels(setup: AcquisitionSetup) -> tuple[int, int]:
+    """(channels, rows) of one detector module."""
+    det = setup.system.detector
+    px = det.nbPixels if isinstance(det, FlatPanelDetector) else det.nbPixelPerModule
+    return int(px[0]), int(px[1])
+
+
 class GeometryEncoder:
     @staticmethod
-    def encodeFullGeometry(setup: AcquisitionSetup) -> np.ndarray:
-        """FP64 projection matrices, shape (views, modules, 3, 4) (geometryencoder.cpp:83-98)."""
+    def encodeFullGeometry(setup: AcquisitionSetup, source_shift=None) -> np.ndarray:
+        """FP64 projection matrices, shape (views, modules, 3, 4) (geometryencoder.cpp:83-98).
+        source_shift: displacement (mm) of the source in its own frame, as
+        prepare::GantryDisplacementParam's source displacement."""
         det = setup.system.detector._c()
         modules = det.modules if det.cylindrical else 1
         V = setup.nbViews()
         out = np.zeros((V, modules, 3, 4), np.float64)
+        shift = _lib.dptr(np.asarray(source_shift, np.float64)) if source_shift is not None else None
         if setup._poses is not None:
             pos, rot = setup._poses
-            check(lib().ctlproj_encode_carm_poses(setup.system.gantry.cArmSpan, C.byref(det),
-                                                  _lib.dptr(pos), _lib.dptr(rot), V,
-                                                  _lib.dptr(out), out.size))
+            check(lib().ctlproj_encode_carm_poses_shifted(setup.system.gantry.cArmSpan, C.byref(det),
+                                                          _lib.dptr(pos), _lib.dptr(rot), V, shift,
+                                                          _lib.dptr(out), out.size))
         else:
             if setup._protocol is None:
                 raise ValueError("no preparation protocol applied")
             g = _gantry(setup.system.gantry)
-            check(lib().ctlproj_encode_trajectory(C.byref(g), C.byref(det), C.byref(setup._protocol.c),
-                                                  V, _lib.dptr(out), out.size))
+            check(lib().ctlproj_encode_trajectory_shifted(C.byref(g), C.byref(det), C.byref(setup._protocol.c),
+                                                          V, shift, _lib.dptr(out), out.size))
         return out
 
 
@@ -452,9 +463,7 @@ class RayCasterProjector:
         """configure(AcquisitionSetup) or configure((pmats, channels, rows))."""
         if isinstance(setup, AcquisitionSetup):
             pm = GeometryEncoder.encodeFullGeometry(setup)
-            det = setup.system.detector
-            px = det.nbPixels if isinstance(det, FlatPanelDetector) else det.nbPixelPerModule
-            self.configure_matrices(pm, px[0], px[1])
+            self.configure_matrices(pm, *_module_pixels(setup))
         else:
             pm, c, r = setup
             self.configure_matrices(pm, c, r)
@@ -529,12 +538,19 @@ class RayCasterProjector:
         return True
 
     def projectComposite(self, volumes) -> ProjectionData:
-        """Sum of the projections of several volumes (abstractprojector.h:99-112)."""
-        total = None
-        for v in volumes:
-            p = self.project(v).data
-            total = p if total is None else total + p
-        return ProjectionData(total)
+        """Sum of the projections of several volumes (abstractprojector.h:99-112), accumulated on
+        the device (ctlproj_project_composite). Host volumes are made resident first."""
+        res = [v if isinstance(v, ResidentVolume) else ResidentVolume(v) for v in volumes]
+        self._push_settings()
+        out = self._out()
+        handles = (C.c_void_p * len(res))(*[r.h.value for r in res])
+        check(lib().ctlproj_project_composite(self.h, handles, len(res), _lib.fptr(out), out.size))
+        return ProjectionData(out)
+
+    def project_composite_device(self, volumes, out_ptr: int, count: int) -> None:
+        self._push_settings()
+        handles = (C.c_void_p * len(volumes))(*[r.h.value for r in volumes])
+        check(lib().ctlproj_project_composite_device(self.h, handles, len(volumes), C.c_void_p(out_ptr), count))
 
     def close(self) -> None:
         if self.h:
@@ -546,6 +562,62 @@ class RayCasterProjector:
             self.close()
         except Exception:
             pass
+
+
+class ArealFocalSpotExtension:
+    """ArealFocalSpotExtension (arealfocalspotextension.cpp:82-166, 257-292): the focal spot of
+    size w x h (CTSystem.focalSpotSize) is sampled on a nx x ny grid over [-1/2, 1/2]^2; each
+    sample shifts the source by (x w, y h, 0) in its own frame. All samples are projected in one
+    launch (the projector configured with the samples' geometries one after the other) and
+    averaged on the device: in the intensity domain (-ln mean exp(-p)), or, with the low-extinction
+    approximation, as extinctions."""
+
+    def __init__(self, projector: "RayCasterProjector | None" = None, discretization=(1, 1),
+                 lowExtinctionApproximation: bool = False):
+        self._nested = projector or RayCasterProjector()
+        self._disc = tuple(int(v) for v in discretization)
+        self._low = bool(lowExtinctionApproximation)
+        self._setup = None
+
+    def use(self, projector: "RayCasterProjector") -> None:
+        self._nested = projector
+
+    def setDiscretization(self, discretization) -> None:
+        self._disc = tuple(int(v) for v in discretization)
+
+    def enableLowExtinctionApproximation(self, enable: bool = True) -> None:
+        self._low = bool(enable)
+
+    def isLinear(self) -> bool:
+        return self._low
+
+    def discretizationGrid(self) -> list:
+        w, h = self._disc
+        xs = [0.0] if w <= 1 else [-0.5 + i / (w - 1) for i in range(w)]
+        ys = [0.0] if h <= 1 else [-0.5 + j / (h - 1) for j in range(h)]
+        return [(x, y) for x in xs for y in ys]  # x outer, as the reference enumerates
+
+    def configure(self, setup: AcquisitionSetup) -> None:
+        self._setup = setup
+        fw, fh = setup.system.focalSpotSize
+        pm = [GeometryEncoder.encodeFullGeometry(setup, (x * fw, y * fh, 0.0))
+              for x, y in self.discretizationGrid()]
+        self._k = len(pm)
+        self._views = pm[0].shape[0]
+        self._nested.configure((np.concatenate(pm, axis=0), *_module_pixels(setup)))
+
+    def project(self, volume) -> ProjectionData:
+        import torch  # device buffers (the library computes; torch only allocates)
+        vol = volume if isinstance(volume, ResidentVolume) else ResidentVolume(volume)
+        V, M, R, C_ = self._nested._dims
+        n = (V // self._k) * M * R * C_
+        buf = torch.empty(self._k * n, dtype=torch.float32, device="cuda")
+        res = torch.empty(n, dtype=torch.float32, device="cuda")
+        self._nested.project_device(vol, buf.data_ptr(), buf.numel())
+        self._nested.synchronize()
+        check(lib().ctlproj_average_projections(self._nested.ctx.h, C.c_void_p(buf.data_ptr()), self._k, n,
+                                                0 if self._low else 1, C.c_void_p(res.data_ptr())))
+        return ProjectionData(res.cpu().numpy().reshape(V // self._k, M, R, C_))
 
 
 class Batch:

@@ -134,6 +134,21 @@ ctlproj_status ctlproj_project_host_volume(ctlproj_projector* p, const float* ho
  * Call ctlproj_projector_synchronize before reading `device_out`. */
 ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume* vol,
                                       float* device_out, size_t out_count);
+/* AbstractProjector::projectComposite (abstractprojector.h:99-112): the sum of the projections of
+ * n resident volumes (any voxel sizes and offsets), accumulated on the device in volume order on
+ * the projector's stream. The host form copies the sum to `out` ([V][M][R][C]). */
+ctlproj_status ctlproj_project_composite(ctlproj_projector* p, const ctlproj_volume* const* vols,
+                                         uint32_t n, float* out, size_t out_count);
+ctlproj_status ctlproj_project_composite_device(ctlproj_projector* p, const ctlproj_volume* const* vols,
+                                                uint32_t n, float* device_out, size_t out_count);
+
+/* Mean of k projections of n values stored one after the other in device memory. intensity_domain:
+ * -ln(mean(exp(-value))) (averaging in the intensity domain with initial intensity 1, as
+ * ArealFocalSpotExtension does, arealfocalspotextension.cpp:82-166); otherwise the plain mean
+ * (its low-extinction approximation). Synchronous on the context's stream. */
+ctlproj_status ctlproj_average_projections(ctlproj_context* ctx, const float* device_in, uint32_t k,
+                                           size_t n, int intensity_domain, float* device_out);
+
 ctlproj_status ctlproj_projector_synchronize(ctlproj_projector* p);
 
 /* Duration of the last ray-casting kernel on the projector's stream (HIP events), in ms. */
@@ -264,6 +279,18 @@ ctlproj_status ctlproj_encode_trajectory(const ctlproj_gantry* gantry, const ctl
 ctlproj_status ctlproj_encode_carm_poses(double carm_span, const ctlproj_detector* det,
                                          const double* positions, const double* rotations,
                                          uint32_t views, double* out, size_t out_count);
+
+/* The same with the source displaced by source_shift (mm, in the source's own frame: the
+ * gantry's source displacement, abstractgantry.h:159-163, as prepare::GantryDisplacementParam
+ * sets it; ArealFocalSpotExtension shifts it over the focal spot). */
+ctlproj_status ctlproj_encode_trajectory_shifted(const ctlproj_gantry* gantry, const ctlproj_detector* det,
+                                                 const ctlproj_trajectory* traj, uint32_t views,
+                                                 const double source_shift[3], double* out,
+                                                 size_t out_count);
+ctlproj_status ctlproj_encode_carm_poses_shifted(double carm_span, const ctlproj_detector* det,
+                                                 const double* positions, const double* rotations,
+                                                 uint32_t views, const double source_shift[3],
+                                                 double* out, size_t out_count);
 
 /* GeometryDecoder::decodeFullGeometry (geometrydecoder.cpp): the physical geometry behind
  * P-matrices [V][M][3][4] of modules with channels x rows pixels. reference: CTLPROJ_DECODE_PIXEL_SIZE

@@ -139,3 +139,21 @@ def test_decoder_rejects_bad_references():
         GeometryDecoder((4, 4), (1.0, 1.0)).decodeFullGeometry(P)
     with pytest.raises(ValueError):
         GeometryDecoder((4, 4))
+
+
+def test_source_shift_moves_the_source_in_its_own_frame():
+    """A source displacement (GantryDisplacementParam, abstractgantry.h:159-163) of a C-arm at pose
+    (p, R) puts the decoded source at p + R shift and leaves the detector where it was."""
+    from ctl_amd import GeometryDecoder
+    rng = np.random.default_rng(11)
+    V, span, pix = 3, 1200.0, (1.0, 1.0)
+    pos = rng.normal(0.0, 200.0, (V, 3))
+    rot = np.stack([np.linalg.qr(rng.normal(size=(3, 3)))[0] for _ in range(V)])
+    rot[np.linalg.det(rot) < 0] *= -1.0
+    setup = AcquisitionSetup(CTSystem(CarmGantry(span), FlatPanelDetector((64, 32), pix)), V)
+    setup.setCarmPoses(pos, rot.reshape(V, 9))
+    shift = np.array([0.7, -0.4, 0.0])
+    dec0 = GeometryDecoder((64, 32), pix).decodeFullGeometry(GeometryEncoder.encodeFullGeometry(setup))
+    dec1 = GeometryDecoder((64, 32), pix).decodeFullGeometry(GeometryEncoder.encodeFullGeometry(setup, shift))
+    assert np.allclose(dec1.source_positions, pos + rot @ shift, rtol=0, atol=1e-9)
+    assert np.allclose(dec1.module_positions, dec0.module_positions, rtol=0, atol=1e-9)

@@ -7,3 +7,4 @@ from .projector import (  # noqa: F401
     ProjectionData, RayCasterProjector, ResidentVolume, Settings, ShortScanTrajectory,
     TubularGantry, VolumeData, VoxelVolume, WobbleTrajectory, ellipsoid_phantom,
 )
+from .processing import RadonTransform3D, SamplingRange, VolumeResampler, VolumeSlicer  # noqa: F401

@@ -154,6 +154,19 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_nrrd_write": (_i, [C.c_char_p, _vp, _i, _u32, C.POINTER(C.c_uint64), C.c_char_p, _f64p, _f64p,
                                 C.c_char_p]),
     "ctlproj_diag_gather_probe": (_i, [_vp, _vp, _i, _vp, _sz, _f32p, _f64p]),
+    "ctlproj_slicer_default_side": (_i, [_u32p, _u32p]),
+    "ctlproj_slice": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _f32p, _sz]),
+    "ctlproj_slice_device": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _vp, _sz]),
+    "ctlproj_resampler_default_ranges": (_i, [_vp, _f32p]),
+    "ctlproj_resample": (_i, [_vp, _f32p, _f32p, _u32, _f32p, _u32, _f32p, _u32, _f32p, _sz]),
+    "ctlproj_sample": (_i, [_vp, _f32p, _f32p, _sz, _f32p]),
+    "ctlproj_sample_device": (_i, [_vp, _f32p, _vp, _sz, _vp]),
+    "ctlproj_radon3d_default_slice": (_i, [_u32p, _f32p, _u32p, _f32p]),
+    "ctlproj_radon3d_side_for_resolution": (_i, [_u32, C.c_float, C.c_float, _u32p]),
+    "ctlproj_radon3d_plane_integrals": (_i, [_vp, _f64p, _u32, _f32p, _u32, _u32, C.c_float, _f32p, _sz]),
+    "ctlproj_radon3d_plane_integrals_device": (_i, [_vp, _f64p, _u32, _f32p, _u32, _u32, C.c_float, _vp, _sz]),
+    "ctlproj_radon3d_sample_transform": (_i, [_vp, _f32p, _u32, _f32p, _u32, _f32p, _u32, _u32, C.c_float,
+                                              _f32p, _sz]),
     "ctlproj_projector_autotune": (_i, [_vp, _vp, _f32p]),
     "ctlproj_autotune_table": (_i, [C.c_char_p, _sz, C.POINTER(_sz)]),
     "ctlproj_batch_create": (_i, [_vp, _pp]),

@@ -200,4 +200,45 @@ hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity,
 
 bool validShape(const LaunchShape& s);
 
+// ---- volume processing (processing.hip) -----------------------------------------------------
+
+// Affine map of a sampling plane, FP32 as the host casts it: sampler coordinate
+//   slices:          u_k = h[4k] (x - cx) + h[4k+1] (y - cy) + h[4k+3] - c[k] + 1/2
+//   plane integrals: u_k = h[4k] x + h[4k+1] y + h[4k+3] + d c[k]   (c = normal / voxel size)
+struct alignas(16) PlaneMap {
+    float h[12];
+    float c[3];
+    float pad;
+};
+static_assert(sizeof(PlaneMap) == 64, "PlaneMap is one 64-byte record");
+
+// Distances per workgroup of the plane-integral kernel.
+constexpr int kPlaneDists = 8;
+
+// One launch of plane integrals: n_orient orientations x n_dist distances over a width x height
+// template plane cut into 8x8 pixel tiles (tiles_x per row, n_tiles in all), each (orientation,
+// distance chunk) split into `splits` tile ranges.
+struct PlaneJob {
+    uint32_t n_orient, n_dist;
+    uint32_t width, height;
+    uint32_t tiles_x, n_tiles;
+    uint32_t splits;
+};
+
+// n_planes slices of width x height pixels into d_out ([plane][y][x]).
+hipError_t launchSlice(const VolumeView& vol, const PlaneMap* d_maps, uint32_t n_planes, uint32_t width,
+                       uint32_t height, float* d_out, hipStream_t s);
+// ranges = {lo1, hi1, lo2, hi2, lo3, hi3}; d_out[k][j][i] at (p1[i], p2[j], p3[k]).
+hipError_t launchResample(const VolumeView& vol, const float ranges[6], const float* d_p1, uint32_t n1,
+                          const float* d_p2, uint32_t n2, const float* d_p3, uint32_t n3, float* d_out,
+                          hipStream_t s);
+// d_out[i] at the point d_xyz[3i .. 3i+2].
+hipError_t launchSamplePoints(const VolumeView& vol, const float ranges[6], const float* d_xyz, size_t n,
+                              float* d_out, hipStream_t s);
+// Plane integrals: d_partial holds n_orient * n_dist * splits floats; d_out[d * out_stride + o]
+// receives scale * (the sum over the template plane).
+hipError_t launchPlaneIntegrals(const VolumeView& vol, const PlaneMap* d_maps, const float* d_dist,
+                                const PlaneJob& job, float* d_partial, float scale, float* d_out,
+                                uint32_t out_stride, hipStream_t s);
+
 } // namespace ctlk

@@ -1,0 +1,263 @@
+// HIP kernels of the volume processing on resident volumes, written for gfx950 (CDNA4, wave64).
+//
+//   slice_kernel     one thread per slice pixel of any number of planes (VolumeSlicer).
+//   resample_kernel  one thread per grid point, sample_kernel one per listed point
+//                    (VolumeResampler).
+//   plane_kernel     plane integrals (RadonTransform3D): a workgroup takes one plane orientation,
+//                    up to kPlaneDists distances and a range of 8x8 pixel tiles of the template
+//                    plane; every wavefront walks its tiles, reading all distances of a pixel
+//                    back to back (the same small patch of the volume, so the gathers share L1
+//                    lines), and keeps one FP32 sum per distance in registers. The workgroup's
+//                    sums go to a partial buffer that plane_reduce_kernel adds up in a fixed
+//                    order, so results do not depend on scheduling.
+//
+// Every read goes through sampleImage: OpenCL's linear sampler with border colour 0 on the padded
+// resident volume, FP32 interpolation for every storage type.
+#include "kernels.h"
+#include "sampler.h"
+
+namespace ctlk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNumXcd = 8;
+
+// Contiguous logical ranges per XCD (workgroup b runs on XCD b % 8), so that neighbouring
+// logical workgroups -- neighbouring orientations in the plane kernel -- share an L2.
+__device__ __forceinline__ uint32_t xcdOrder(uint32_t b, uint32_t nb)
+{
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    const uint32_t x = b % kNumXcd, k = b / kNumXcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ Pitch pitchOf(const VolumeView& v)
+{
+    const uint32_t eb = v.layout == Layout::Interleaved ? 8u : (v.storage == Storage::F32 ? 4u : 2u);
+    Pitch p;
+    p.ty = v.pitch_y;
+    p.tz = v.pitch_z;
+    p.y = v.pitch_y * eb;
+    p.z = v.pitch_z * eb;
+    p.yz = (v.pitch_y + v.pitch_z) * eb;
+    return p;
+}
+
+__device__ __forceinline__ float waveSum(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// volumeSlicer.cl: u = H (x - cx, y - cy, 0, 1)^T - corner + 1/2 per plane; out[plane][y][x].
+template <Storage ST, Layout LY>
+__global__ __launch_bounds__(kBlock) void slice_kernel(VolumeView vol, const PlaneMap* __restrict__ maps,
+                                                       uint32_t width, uint32_t height, uint32_t tiles_x,
+                                                       uint32_t tiles_per_plane, float cx, float cy,
+                                                       float* __restrict__ out)
+{
+    const uint32_t plane = blockIdx.x / tiles_per_plane, t = blockIdx.x % tiles_per_plane;
+    const uint32_t x = (t % tiles_x) * 16 + (threadIdx.x & 15), y = (t / tiles_x) * 16 + (threadIdx.x >> 4);
+    if (x >= width || y >= height) return;
+    const PlaneMap& m = maps[plane];
+    const float vx = float(x) - cx, vy = float(y) - cy;
+    float u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) u[k] = fmaf(m.h[4 * k], vx, fmaf(m.h[4 * k + 1], vy, m.h[4 * k + 3])) - m.c[k] + 0.5f;
+    const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    out[(size_t(plane) * height + y) * width + x] =
+        sampleImage<ST, LY>(makeRsrc(vol.data, vol.bytes), pitchOf(vol), nhi, u[0], u[1], u[2]);
+}
+
+// volumeResampler.cl: u = (N - 1) / (hi - lo) * (p - lo) + 1/2 per axis.
+struct Ranges {
+    float lo[3], scale[3];
+};
+
+template <Storage ST, Layout LY>
+__global__ __launch_bounds__(kBlock) void resample_kernel(VolumeView vol, Ranges r, const float* __restrict__ p1,
+                                                          const float* __restrict__ p2, const float* __restrict__ p3,
+                                                          uint32_t n1, uint32_t n2, uint32_t count,
+                                                          float* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t s1 = i % n1, s23 = i / n1, s2 = s23 % n2, s3 = s23 / n2;
+    const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    out[i] = sampleImage<ST, LY>(makeRsrc(vol.data, vol.bytes), pitchOf(vol), nhi,
+                                 r.scale[0] * (p1[s1] - r.lo[0]) + 0.5f, r.scale[1] * (p2[s2] - r.lo[1]) + 0.5f,
+                                 r.scale[2] * (p3[s3] - r.lo[2]) + 0.5f);
+}
+
+template <Storage ST, Layout LY>
+__global__ __launch_bounds__(kBlock) void sample_kernel(VolumeView vol, Ranges r, const float* __restrict__ xyz,
+                                                        size_t count, float* __restrict__ out)
+{
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const float* p = xyz + 3 * i;
+    const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    out[i] = sampleImage<ST, LY>(makeRsrc(vol.data, vol.bytes), pitchOf(vol), nhi,
+                                 r.scale[0] * (p[0] - r.lo[0]) + 0.5f, r.scale[1] * (p[1] - r.lo[1]) + 0.5f,
+                                 r.scale[2] * (p[2] - r.lo[2]) + 0.5f);
+}
+
+// planeIntegral.cl: u = H (x, y, 0, 1)^T + d * shift for pixel (x, y) and distance d.
+// Logical workgroup b = (distance chunk, orientation, split), orientation fastest after the split.
+template <Storage ST, Layout LY>
+__global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const PlaneMap* __restrict__ maps,
+                                                       const float* __restrict__ dist, PlaneJob job,
+                                                       float* __restrict__ partial)
+{
+    const uint32_t b = xcdOrder(blockIdx.x, gridDim.x);
+    const uint32_t split = b % job.splits, rest = b / job.splits;
+    const uint32_t o = rest % job.n_orient, d0 = (rest / job.n_orient) * kPlaneDists;
+    const uint32_t nd = min(uint32_t(kPlaneDists), job.n_dist - d0);
+    const uint32_t t0 = uint32_t(uint64_t(job.n_tiles) * split / job.splits);
+    const uint32_t t1 = uint32_t(uint64_t(job.n_tiles) * (split + 1) / job.splits);
+    const PlaneMap& m = maps[o];
+    const float h00 = m.h[0], h01 = m.h[1], h03 = m.h[3];
+    const float h10 = m.h[4], h11 = m.h[5], h13 = m.h[7];
+    const float h20 = m.h[8], h21 = m.h[9], h23 = m.h[11];
+    const float s0 = m.c[0], s1 = m.c[1], s2 = m.c[2];
+    float dd[kPlaneDists], acc[kPlaneDists];
+#pragma unroll
+    for (int j = 0; j < kPlaneDists; ++j) {
+        dd[j] = uint32_t(j) < nd ? dist[d0 + j] : 0.0f;
+        acc[j] = 0.0f;
+    }
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+    const Pitch p = pitchOf(vol);
+    const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t t = t0 + wave; t < t1; t += kBlock / 64) {
+        const uint32_t x = (t % job.tiles_x) * 8 + (lane & 7), y = (t / job.tiles_x) * 8 + (lane >> 3);
+        if (x >= job.width || y >= job.height) continue;
+        const float fx = float(x), fy = float(y);
+        const float b0 = fmaf(h00, fx, fmaf(h01, fy, h03));
+        const float b1 = fmaf(h10, fx, fmaf(h11, fy, h13));
+        const float b2 = fmaf(h20, fx, fmaf(h21, fy, h23));
+#pragma unroll
+        for (int j = 0; j < kPlaneDists; ++j)
+            if (uint32_t(j) < nd)
+                acc[j] += sampleImage<ST, LY>(rs, p, nhi, fmaf(dd[j], s0, b0), fmaf(dd[j], s1, b1),
+                                              fmaf(dd[j], s2, b2));
+    }
+    __shared__ float red[kBlock / 64][kPlaneDists];
+#pragma unroll
+    for (int j = 0; j < kPlaneDists; ++j) {
+        const float v = waveSum(acc[j]);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nd) {
+        const uint32_t j = threadIdx.x;
+        const float s = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+        partial[(size_t(o) * job.n_dist + d0 + j) * job.splits + split] = s;
+    }
+}
+
+// out[d * out_stride + o] = scale * (sum of the splits of (o, d), in split order).
+__global__ __launch_bounds__(kBlock) void plane_reduce_kernel(const float* __restrict__ partial, uint32_t n_orient,
+                                                              uint32_t n_dist, uint32_t splits, float scale,
+                                                              float* __restrict__ out, uint32_t out_stride)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_orient * n_dist) return;
+    const uint32_t o = i % n_orient, d = i / n_orient;
+    const float* src = partial + (size_t(o) * n_dist + d) * splits;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < splits; ++k) s += src[k];
+    out[size_t(d) * out_stride + o] = s * scale;
+}
+
+#define CTLK_DISPATCH(KERNEL, vol, grid, ...)                                                       \
+    do {                                                                                           \
+        const bool il_ = (vol).layout == Layout::Interleaved;                                      \
+        switch ((vol).storage) {                                                                   \
+        case Storage::F16:                                                                         \
+            if (il_) KERNEL<Storage::F16, Layout::Interleaved><<<grid, kBlock, 0, s>>>(__VA_ARGS__); \
+            else KERNEL<Storage::F16, Layout::Plain><<<grid, kBlock, 0, s>>>(__VA_ARGS__);         \
+            break;                                                                                 \
+        case Storage::BF16:                                                                        \
+            if (il_) KERNEL<Storage::BF16, Layout::Interleaved><<<grid, kBlock, 0, s>>>(__VA_ARGS__); \
+            else KERNEL<Storage::BF16, Layout::Plain><<<grid, kBlock, 0, s>>>(__VA_ARGS__);        \
+            break;                                                                                 \
+        default:                                                                                   \
+            if (il_) KERNEL<Storage::F32, Layout::Interleaved><<<grid, kBlock, 0, s>>>(__VA_ARGS__); \
+            else KERNEL<Storage::F32, Layout::Plain><<<grid, kBlock, 0, s>>>(__VA_ARGS__);         \
+            break;                                                                                 \
+        }                                                                                          \
+    } while (0)
+
+Ranges makeRanges(const VolumeView& vol, const float ranges[6])
+{
+    Ranges r;
+    for (int k = 0; k < 3; ++k) {
+        r.lo[k] = ranges[2 * k];
+        r.scale[k] = (float(vol.n[k]) - 1.0f) / (ranges[2 * k + 1] - ranges[2 * k]);
+    }
+    return r;
+}
+
+} // namespace
+
+hipError_t launchSlice(const VolumeView& vol, const PlaneMap* d_maps, uint32_t n_planes, uint32_t width,
+                       uint32_t height, float* d_out, hipStream_t s)
+{
+    const uint32_t tiles_x = (width + 15) / 16, tiles_y = (height + 15) / 16;
+    const uint64_t per_plane = uint64_t(tiles_x) * tiles_y, blocks = per_plane * n_planes;
+    if (blocks == 0) return hipSuccess;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    const float cx = 0.5f * float(width - 1u), cy = 0.5f * float(height - 1u);
+    CTLK_DISPATCH(slice_kernel, vol, uint32_t(blocks), vol, d_maps, width, height, tiles_x, uint32_t(per_plane),
+                  cx, cy, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchResample(const VolumeView& vol, const float ranges[6], const float* d_p1, uint32_t n1,
+                          const float* d_p2, uint32_t n2, const float* d_p3, uint32_t n3, float* d_out,
+                          hipStream_t s)
+{
+    const uint64_t count = uint64_t(n1) * n2 * n3;
+    if (count == 0) return hipSuccess;
+    if (count > 0xffffffffull - kBlock) return hipErrorInvalidConfiguration;
+    const Ranges r = makeRanges(vol, ranges);
+    const uint32_t blocks = uint32_t((count + kBlock - 1) / kBlock);
+    CTLK_DISPATCH(resample_kernel, vol, blocks, vol, r, d_p1, d_p2, d_p3, n1, n2, uint32_t(count), d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchSamplePoints(const VolumeView& vol, const float ranges[6], const float* d_xyz, size_t n,
+                              float* d_out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = (uint64_t(n) + kBlock - 1) / kBlock;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    const Ranges r = makeRanges(vol, ranges);
+    CTLK_DISPATCH(sample_kernel, vol, uint32_t(blocks), vol, r, d_xyz, n, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchPlaneIntegrals(const VolumeView& vol, const PlaneMap* d_maps, const float* d_dist,
+                                const PlaneJob& job, float* d_partial, float scale, float* d_out,
+                                uint32_t out_stride, hipStream_t s)
+{
+    if (job.n_orient == 0 || job.n_dist == 0) return hipSuccess;
+    if (job.splits == 0 || job.tiles_x == 0 || job.n_tiles == 0) return hipErrorInvalidValue;
+    const uint64_t chunks = (uint64_t(job.n_dist) + kPlaneDists - 1) / kPlaneDists;
+    const uint64_t blocks = chunks * job.n_orient * job.splits;
+    if (blocks > 0x7fffffffull || uint64_t(job.n_orient) * job.n_dist > 0xffffffffull - kBlock)
+        return hipErrorInvalidConfiguration;
+    CTLK_DISPATCH(plane_kernel, vol, uint32_t(blocks), vol, d_maps, d_dist, job, d_partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t rb = uint32_t((uint64_t(job.n_orient) * job.n_dist + kBlock - 1) / kBlock);
+    plane_reduce_kernel<<<rb, kBlock, 0, s>>>(d_partial, job.n_orient, job.n_dist, job.splits, scale, d_out,
+                                              out_stride);
+    return hipGetLastError();
+}
+
+} // namespace ctlk

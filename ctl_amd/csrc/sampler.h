@@ -1,0 +1,168 @@
+// Texel samplers of resident volumes (device code shared by the projector and the volume
+// processing kernels).
+//
+// A resident volume is padded by one zero texel on every side (VolumeView). The OpenCL image
+// sampler the reference reads volumes with (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP, border 0,
+// unnormalised coordinate u) interpolates the voxels around u - 1/2; in the padded frame that is
+// the texel position q = u + 1/2, whose taps are texels floor(q) and floor(q) + 1.
+#pragma once
+
+#include "kernels.h"
+
+#include <cstdint>
+
+namespace ctlk {
+
+namespace {
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// buffer resource word 3 for gfx9-family raw buffers
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t makeRsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, kRsrcWord3);
+}
+
+// ---------------------------------------------------------------------------------------------
+// trilinear / nearest sample at texel-frame position q (padded frame, blo <= q < n+1)
+
+__device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+__device__ __forceinline__ float halfLo(uint32_t u) { return float(__builtin_bit_cast(half2_t, u).x); }
+__device__ __forceinline__ float halfHi(uint32_t u) { return float(__builtin_bit_cast(half2_t, u).y); }
+__device__ __forceinline__ float bfLo(uint32_t u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bfHi(uint32_t u) { return __uint_as_float(u & 0xffff0000u); }
+
+// x-lerp of a packed pair (a = element at x, b = element at x+1) in FP32 arithmetic; for FP16 the
+// two FMAs read the halves directly (v_fma_mix).
+template <Storage ST>
+__device__ __forceinline__ float lerpPair(uint32_t lo_word, bool lo_hi, uint32_t hi_word, bool hi_hi, float w)
+{
+    if constexpr (ST == Storage::F16) {
+        const float a = lo_hi ? halfHi(lo_word) : halfLo(lo_word);
+        const float b = hi_hi ? halfHi(hi_word) : halfLo(hi_word);
+        return fmaf(w, b, fmaf(-w, a, a));
+    } else {
+        const float a = lo_hi ? bfHi(lo_word) : bfLo(lo_word);
+        const float b = hi_hi ? bfHi(hi_word) : bfLo(hi_word);
+        return lerp(a, b, w);
+    }
+}
+
+template <Storage ST, Layout LY> struct Sampler;
+
+// FP32, one voxel per texel: four 8-byte loads (x-pairs at y/z corners).
+template <> struct Sampler<Storage::F32, Layout::Plain> {
+    static constexpr uint32_t kShift = 2;
+    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    {
+        const float2 t = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0));
+        return lerp(t.x, t.y, w);
+    }
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const float c00 = lerpx(r, ofs, 0, wx), c10 = lerpx(r, ofs, p.y, wx);
+        const float c01 = lerpx(r, ofs, p.z, wx), c11 = lerpx(r, ofs, p.yz, wx);
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+
+// FP16/BF16, one voxel per texel: four 4-byte loads.
+template <Storage ST> struct PlainHalf {
+    static constexpr uint32_t kShift = 1;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const uint32_t t00 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, 0, 0);
+        const uint32_t t10 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.y, 0);
+        const uint32_t t01 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.z, 0);
+        const uint32_t t11 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.yz, 0);
+        const float c00 = lerpPair<ST>(t00, false, t00, true, wx);
+        const float c10 = lerpPair<ST>(t10, false, t10, true, wx);
+        const float c01 = lerpPair<ST>(t01, false, t01, true, wx);
+        const float c11 = lerpPair<ST>(t11, false, t11, true, wx);
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+template <> struct Sampler<Storage::F16, Layout::Plain> : PlainHalf<Storage::F16> {};
+template <> struct Sampler<Storage::BF16, Layout::Plain> : PlainHalf<Storage::BF16> {};
+
+// FP32 z-pair texels: two 16-byte loads, rows y and y+1, each [v(x,z) v(x,z+1) v(x+1,z) v(x+1,z+1)].
+template <> struct Sampler<Storage::F32, Layout::Interleaved> {
+    static constexpr uint32_t kShift = 3;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+        const float4 b = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, p.y, 0));
+        const float c00 = lerp(a.x, a.z, wx), c01 = lerp(a.y, a.w, wx); // y0: z0, z1
+        const float c10 = lerp(b.x, b.z, wx), c11 = lerp(b.y, b.w, wx); // y1: z0, z1
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+
+// FP16/BF16 (y,z)-quad texels: one 16-byte load = the quads at x and x+1, each
+// [v(y,z) v(y+1,z) | v(y,z+1) v(y+1,z+1)] as two 32-bit words.
+template <Storage ST> struct QuadHalf {
+    static constexpr uint32_t kShift = 3;
+    static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch&, uint32_t ofs,
+                                                float wx, float wy, float wz)
+    {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        const u4 t = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
+        const float c00 = lerpPair<ST>(t.x, false, t.z, false, wx); // y0 z0
+        const float c10 = lerpPair<ST>(t.x, true, t.z, true, wx);   // y1 z0
+        const float c01 = lerpPair<ST>(t.y, false, t.w, false, wx); // y0 z1
+        const float c11 = lerpPair<ST>(t.y, true, t.w, true, wx);   // y1 z1
+        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    }
+};
+template <> struct Sampler<Storage::F16, Layout::Interleaved> : QuadHalf<Storage::F16> {};
+template <> struct Sampler<Storage::BF16, Layout::Interleaved> : QuadHalf<Storage::BF16> {};
+
+template <Storage ST, Layout LY>
+__device__ __forceinline__ float trilinear(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
+                                           float qy, float qz)
+{
+    const int ix = int(qx), iy = int(qy), iz = int(qz); // q >= 0: truncation is floor
+    const float wx = __builtin_amdgcn_fractf(qx);
+    const float wy = __builtin_amdgcn_fractf(qy);
+    const float wz = __builtin_amdgcn_fractf(qz);
+    const uint32_t idx = uint32_t(ix) + __umul24(uint32_t(iy), p.ty) + __umul24(uint32_t(iz), p.tz);
+    return Sampler<ST, LY>::tri(r, p, idx << Sampler<ST, LY>::kShift, wx, wy, wz);
+}
+
+template <Storage ST, Layout LY>
+__device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& p, float qx,
+                                         float qy, float qz)
+{
+    const uint32_t idx = uint32_t(int(qx)) + __umul24(uint32_t(int(qy)), p.ty) +
+                         __umul24(uint32_t(int(qz)), p.tz);
+    constexpr uint32_t shift = LY == Layout::Interleaved ? 3 : (ST == Storage::F32 ? 2 : 1);
+    if constexpr (ST == Storage::F32) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, idx << shift, 0, 0));
+    } else {
+        const uint32_t raw = __builtin_amdgcn_raw_buffer_load_b16(r, idx << shift, 0, 0);
+        if constexpr (ST == Storage::F16) return float(__builtin_bit_cast(_Float16, uint16_t(raw)));
+        else return __uint_as_float(raw << 16);
+    }
+}
+
+// OpenCL linear image read at unnormalised coordinate u (the reference's read_imagef with
+// CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP): zero wherever all eight taps are border texels, that is
+// unless 0 <= u + 1/2 < n + 1 on every axis (nhi = n + 1). NaN coordinates read zero.
+template <Storage ST, Layout LY>
+__device__ __forceinline__ float sampleImage(__amdgpu_buffer_rsrc_t rs, const Pitch& p, const float nhi[3],
+                                             float ux, float uy, float uz)
+{
+    const float qx = ux + 0.5f, qy = uy + 0.5f, qz = uz + 0.5f;
+    if (!(qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2]))
+        return 0.0f;
+    return trilinear<ST, LY>(rs, p, qx, qy, qz);
+}
+
+} // namespace
+} // namespace ctlk

@@ -341,6 +341,7 @@ class ResidentVolume:
         self.h = C.c_void_p()
         self.volume = volume
         self.storage = storage
+        self.dims, self.voxel_size, self.offset = volume.dims, volume.voxel_size, volume.offset
         check(lib().ctlproj_volume_create(
             self.ctx.h, _lib.fptr(volume.data), _lib.u32arr(*volume.dims),
             _lib.f32arr(*volume.voxel_size), _lib.f32arr(*volume.offset), STORAGE[storage],
@@ -354,6 +355,9 @@ class ResidentVolume:
         self.h = C.c_void_p()
         self.volume = None
         self.storage = storage
+        self.dims = tuple(int(v) for v in dims)
+        self.voxel_size = tuple(float(v) for v in voxel_size)
+        self.offset = tuple(float(v) for v in offset)
         check(lib().ctlproj_volume_create_from_device(
             self.ctx.h, C.c_void_p(ptr), _lib.u32arr(*dims), _lib.f32arr(*voxel_size),
             _lib.f32arr(*offset), STORAGE[storage], C.byref(self.h)))

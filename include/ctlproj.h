@@ -379,6 +379,72 @@ ctlproj_status ctlproj_nrrd_write(const char* path, const void* data, int elem, 
                                   const uint64_t* sizes, const char* labels, const double* spacing,
                                   const double* origin, const char* keyvals);
 
+/* ---- volume processing on resident volumes ----------------------------------------------
+ * VolumeSlicer (processing/volumeslicer.h:15-60), VolumeResampler (processing/volumeresampler.h:
+ * 13-55) and RadonTransform3D (processing/radontransform3d.h:16-90) upload a volume into an
+ * OpenCL image and read it with the linear sampler (CLK_FILTER_LINEAR, CLK_ADDRESS_CLAMP,
+ * border colour 0). These entry points read a resident volume of any storage type the same way,
+ * with FP32 interpolation. Plane geometry is computed in FP64 and cast to FP32 as the reference's
+ * host code does. Every call is synchronous on the volume's context stream; device arrays
+ * belong to the volume's device. */
+
+/* VolumeSlicer::sliceDim (volumeslicer.cpp:224-229): default slice side ceil(1.4142 * max(dims))
+ * pixels; the default resolution is the smallest voxel size. */
+ctlproj_status ctlproj_slicer_default_side(const uint32_t dims[3], uint32_t* side);
+/* VolumeSlicer::slice (volumeslicer.cpp:107-178) for n_planes planes at once: plane k is
+ * {p : normals[3k..3k+2] . p = distances[k]} (unit normal, mm), sampled on width x height pixels of
+ * `resolution` mm centred on distances[k] * normal. out / device_out: n_planes*height*width floats,
+ * [plane][y][x]. Non-unit normals and resolution <= 0 return CTLPROJ_ERR_INVALID_ARGUMENT. */
+ctlproj_status ctlproj_slice(const ctlproj_volume* vol, const double* normals, const double* distances,
+                             uint32_t n_planes, uint32_t width, uint32_t height, float resolution, float* out,
+                             size_t out_count);
+ctlproj_status ctlproj_slice_device(const ctlproj_volume* vol, const double* normals, const double* distances,
+                                    uint32_t n_planes, uint32_t width, uint32_t height, float resolution,
+                                    float* device_out, size_t out_count);
+
+/* VolumeResampler's default sampling ranges (volumeresampler.cpp:74-83): per axis
+ * {offset - voxel*(N-1)/2, offset + voxel*(N-1)/2}, i.e. the first and last voxel centres.
+ * ranges = {lo1, hi1, lo2, hi2, lo3, hi3}. */
+ctlproj_status ctlproj_resampler_default_ranges(const ctlproj_volume* vol, float ranges[6]);
+/* VolumeResampler::resample (volumeresampler.cpp:117-171): the ranges map onto the voxel centres
+ * 0 .. N-1 of each axis; out holds n1*n2*n3 floats, out[i + j*n1 + k*n1*n2] at (p1[i], p2[j], p3[k]). */
+ctlproj_status ctlproj_resample(const ctlproj_volume* vol, const float ranges[6], const float* p1, uint32_t n1,
+                                const float* p2, uint32_t n2, const float* p3, uint32_t n3, float* out,
+                                size_t out_count);
+/* VolumeResampler::sample (volumeresampler.cpp:173-213; the cl::Buffer overload :215-255 is the
+ * device form): out[i] at the point xyz[3i .. 3i+2]. */
+ctlproj_status ctlproj_sample(const ctlproj_volume* vol, const float ranges[6], const float* xyz, size_t n,
+                              float* out);
+ctlproj_status ctlproj_sample_device(const ctlproj_volume* vol, const float ranges[6], const float* device_xyz,
+                                     size_t n, float* device_out);
+
+/* RadonTransform3D's default template plane (radontransform3d.cpp:18-24, 278-301): side =
+ * ceil(1.4142 * max(dims)) rounded up to a multiple of 16 pixels, resolution = smallest voxel. */
+ctlproj_status ctlproj_radon3d_default_slice(const uint32_t dims[3], const float voxel_mm[3], uint32_t* side,
+                                             float* resolution);
+/* RadonTransform3D::setSliceResolution (radontransform3d.cpp:239-251): the side that keeps the
+ * template plane's extent at a new resolution (a multiple of 16). */
+ctlproj_status ctlproj_radon3d_side_for_resolution(uint32_t side, float resolution, float new_resolution,
+                                                   uint32_t* new_side);
+/* Plane integrals (RadonTransform3D::planeIntegral, radontransform3d.cpp:188-216, and the device
+ * loop of sampleTransform, :55-156): out[d*n_orient + o] = integral of the volume over the plane
+ * {p : normals[o] . p = distances[d]}, sampled on a side x side template plane of `resolution` mm
+ * (sum of the samples times resolution^2). */
+ctlproj_status ctlproj_radon3d_plane_integrals(const ctlproj_volume* vol, const double* normals,
+                                               uint32_t n_orient, const float* distances, uint32_t n_dist,
+                                               uint32_t side, float resolution, float* out, size_t out_count);
+ctlproj_status ctlproj_radon3d_plane_integrals_device(const ctlproj_volume* vol, const double* normals,
+                                                      uint32_t n_orient, const float* distances, uint32_t n_dist,
+                                                      uint32_t side, float resolution, float* device_out,
+                                                      size_t out_count);
+/* RadonTransform3D::sampleTransform (radontransform3d.cpp:55-156): the grid of azimuth x polar x
+ * distance samples, normal (sin(polar) cos(azimuth), sin(polar) sin(azimuth), cos(polar));
+ * out[a + n_azi*(p + n_pol*d)] (the returned VoxelVolume's x-fastest order). */
+ctlproj_status ctlproj_radon3d_sample_transform(const ctlproj_volume* vol, const float* azimuth, uint32_t n_azi,
+                                                const float* polar, uint32_t n_pol, const float* distances,
+                                                uint32_t n_dist, uint32_t side, float resolution, float* out,
+                                                size_t out_count);
+
 /* ---- gather-ceiling probes (diagnostics) -------------------------------------------------
  * No counterpart in the reference: they measure the ceiling against which the ray caster's gather
  * rate is judged, and compute no projection.

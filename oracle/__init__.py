@@ -56,6 +56,16 @@ def lib():
         L.oracle_sample_linear64.argtypes = [f, u, C.c_double, C.c_double, C.c_double]
         L.oracle_sample_linear64.restype = C.c_double
         L.oracle_clip32.argtypes = [f, f, f, f, C.c_int, f]
+        L.oracle_slice.argtypes = [f, u, f, f, d, C.c_double, C.c_uint32, C.c_uint32, C.c_float,
+                                   C.c_int, f]
+        L.oracle_slice.restype = C.c_int
+        L.oracle_resample.argtypes = [f, u, f, f, C.c_uint32, f, C.c_uint32, f, C.c_uint32, C.c_int, f]
+        L.oracle_resample.restype = C.c_int
+        L.oracle_sample_points.argtypes = [f, u, f, f, C.c_size_t, C.c_int, f]
+        L.oracle_sample_points.restype = C.c_int
+        L.oracle_plane_integrals.argtypes = [f, u, f, f, d, C.c_uint32, f, C.c_uint32, C.c_uint32,
+                                             C.c_uint32, C.c_float, C.c_int, f]
+        L.oracle_plane_integrals.restype = C.c_int
         _lib = L
     return _lib
 
@@ -109,6 +119,61 @@ def sample_linear64(vol, px, py, pz):
     z, y, x = v.shape
     dims = (C.c_uint32 * 3)(x, y, z)
     return lib().oracle_sample_linear64(_f(v), dims, px, py, pz)
+
+
+def _vol_args(vol):
+    v = np.ascontiguousarray(vol, np.float32)
+    z, y, x = v.shape
+    return v, (C.c_uint32 * 3)(x, y, z)
+
+
+def slice_plane(vol, voxel_mm, offset_mm, normal, distance, width, height, resolution, fp64=False):
+    """VolumeSlicer::slice; vol is a (Z, Y, X) array, returns (height, width)."""
+    v, dims = _vol_args(vol)
+    out = np.zeros((height, width), np.float32)
+    vs, off = np.asarray(voxel_mm, np.float32), np.asarray(offset_mm, np.float32)
+    n = np.asarray(normal, np.float64)
+    st = lib().oracle_slice(_f(v), dims, _f(vs), _f(off), _d(n), float(distance), width, height,
+                            float(resolution), int(fp64), _f(out))
+    assert st == 0
+    return out
+
+
+def resample(vol, ranges, p1, p2, p3, fp64=False):
+    """VolumeResampler::resample; returns (len(p3), len(p2), len(p1))."""
+    v, dims = _vol_args(vol)
+    r = np.asarray(ranges, np.float32).reshape(6)
+    a = [np.ascontiguousarray(p, np.float32) for p in (p1, p2, p3)]
+    out = np.zeros((len(a[2]), len(a[1]), len(a[0])), np.float32)
+    st = lib().oracle_resample(_f(v), dims, _f(r), _f(a[0]), len(a[0]), _f(a[1]), len(a[1]),
+                               _f(a[2]), len(a[2]), int(fp64), _f(out))
+    assert st == 0
+    return out
+
+
+def sample_points(vol, ranges, xyz, fp64=False):
+    """VolumeResampler::sample; xyz is (n, 3), returns (n,)."""
+    v, dims = _vol_args(vol)
+    r = np.asarray(ranges, np.float32).reshape(6)
+    pts = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float32)
+    st = lib().oracle_sample_points(_f(v), dims, _f(r), _f(pts), len(pts), int(fp64), _f(out))
+    assert st == 0
+    return out
+
+
+def plane_integrals(vol, voxel_mm, offset_mm, normals, distances, width, height, resolution,
+                    fp64=False):
+    """RadonTransform3D plane integrals; normals (n_orient, 3), returns (n_dist, n_orient)."""
+    v, dims = _vol_args(vol)
+    vs, off = np.asarray(voxel_mm, np.float32), np.asarray(offset_mm, np.float32)
+    n = np.ascontiguousarray(normals, np.float64).reshape(-1, 3)
+    dist = np.ascontiguousarray(distances, np.float32).reshape(-1)
+    out = np.zeros((len(dist), len(n)), np.float32)
+    st = lib().oracle_plane_integrals(_f(v), dims, _f(vs), _f(off), _d(n), len(n), _f(dist), len(dist),
+                                      width, height, float(resolution), int(fp64), _f(out))
+    assert st == 0
+    return out
 
 
 def clip32(source, direction, vol_size, vol_corner, interpolate=True):

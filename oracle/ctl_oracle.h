@@ -17,6 +17,7 @@
 #ifndef CTL_ORACLE_H
 #define CTL_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -70,6 +71,30 @@ double oracle_sample_linear64(const float* vol, const uint32_t dims[3], double p
 /* ref32 slab clip (raycasterprojector_interp.cl:88-148); writes (lambda_min, lambda_max). */
 void oracle_clip32(const float source[3], const float direction[3], const float vol_size[3],
                    const float vol_corner[3], int interpolate, float minmax[2]);
+
+/* ---- volume processing (ctl_oracle_processing.c); fp64: 0 -> ref32, 1 -> ref64 ---------- */
+
+/* VolumeSlicer::slice (modules/src/processing/volumeslicer.cpp:107-161, 204-222 and
+ * modules/src/ocl/cl_src/processing/volumeSlicer.cl): the plane {p : normal . p = distance},
+ * width x height pixels of `resolution` mm centred on distance * normal; out[y][x]. */
+int oracle_slice(const float* vol, const uint32_t dims[3], const float voxel_mm[3],
+                 const float offset_mm[3], const double normal[3], double distance, uint32_t width,
+                 uint32_t height, float resolution, int fp64, float* out);
+/* VolumeResampler::resample / sample (volumeresampler.cpp:117-213, volumeResampler.cl):
+ * ranges = {lo1, hi1, lo2, hi2, lo3, hi3} map onto the voxel centres 0 .. N-1 of each axis.
+ * resample: out[k][j][i] at (p1[i], p2[j], p3[k]); sample: out[i] at xyz[3i .. 3i+2]. */
+int oracle_resample(const float* vol, const uint32_t dims[3], const float ranges[6],
+                    const float* p1, uint32_t n1, const float* p2, uint32_t n2, const float* p3,
+                    uint32_t n3, int fp64, float* out);
+int oracle_sample_points(const float* vol, const uint32_t dims[3], const float ranges[6],
+                         const float* xyz, size_t n, int fp64, float* out);
+/* RadonTransform3D plane integrals (radontransform3d.cpp:55-156, 188-216, 303-450,
+ * planeIntegral.cl): out[d][o] = integral over the plane {p : normals[o] . p = distances[d]}
+ * sampled on a width x height grid of `resolution` mm. */
+int oracle_plane_integrals(const float* vol, const uint32_t dims[3], const float voxel_mm[3],
+                           const float offset_mm[3], const double* normals, uint32_t n_orient,
+                           const float* distances, uint32_t n_dist, uint32_t width, uint32_t height,
+                           float resolution, int fp64, float* out);
 
 #ifdef __cplusplus
 }

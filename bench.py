@@ -269,6 +269,101 @@ def bench_served(args, torch, W, ctl):
     return res
 
 
+def bench_processing(args, torch, w, ctl, oracle):
+    """Volume processing on the resident 256^3 phantom of the headline workload (FP32):
+    RadonTransform3D plane integrals (32 x 32 orientations x 128 distances over +-90 mm on the
+    default 368^2 template plane of 0.5 mm pixels), 256 VolumeSlicer planes of 363^2 pixels and
+    16 Mi VolumeResampler points, every output in device memory."""
+    import ctypes as C
+    from ctl_amd import _lib
+    L = _lib.lib()
+    vol = ctl.VoxelVolume(w.volume, w.voxel_mm, w.offset_mm)
+    rt = ctl.RadonTransform3D(vol)
+    side, reso = rt.sliceDimensions()[0], rt.sliceResolution()
+    azi = np.linspace(0.0, np.pi, 32, endpoint=False)
+    pol = np.linspace(0.02, np.pi - 0.02, 32)
+    normals = np.ascontiguousarray([[np.sin(b) * np.cos(a), np.sin(b) * np.sin(a), np.cos(b)]
+                                    for b in pol for a in azi])
+    dist = np.linspace(-90.0, 90.0, 128).astype(np.float32)
+    out = torch.empty(len(normals) * len(dist), dtype=torch.float32, device="cuda")
+
+    def planes(h):
+        _lib.check(L.ctlproj_radon3d_plane_integrals_device(h, _lib.dptr(normals), len(normals),
+                                                            _lib.fptr(dist), len(dist), side, reso,
+                                                            C.c_void_p(out.data_ptr()), out.numel()))
+
+    planes(rt._vol.h)
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        planes(rt._vol.h)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    samples = len(normals) * len(dist) * side * side
+    # samples that read the volume: the plane integrals of a volume of ones / pixel area
+    ones = ctl.RadonTransform3D(ctl.VoxelVolume(np.ones_like(w.volume), w.voxel_mm, w.offset_mm))
+    planes(ones._vol.h)
+    inside = float(out.double().sum().item()) / (reso * reso)
+    gather = inside * 32 / wall / 1e9
+    res = {"radon3d": dict(
+        orientations=len(normals), distances=len(dist), side=side, resolution_mm=reso, ms=wall * 1e3,
+        template_samples=samples, template_gsamples_s=samples / wall / 1e9,
+        inside_samples=inside, inside_gsamples_s=inside / wall / 1e9,
+        roofline={"bound": "l1", "achieved": gather, "peak": L1_PEAK_GBPS, "unit": "GB/s",
+                  "frac": gather / L1_PEAK_GBPS,
+                  "note": "achieved = samples inside the support x 8 taps x 4 B per wall second; "
+                          "the kernel is bound by cache-line traffic beyond L2 (counters in "
+                          "profiles/plane_kernel_f32_256cube_1024orient_128dist.md)"})}
+    log(f"radon3d: {json.dumps(res['radon3d'])}")
+    del ones
+    if oracle is not None and not args.no_cpu:
+        pick = normals[::128]  # 8 orientations spread over the sphere
+        t0 = time.perf_counter()
+        oracle.plane_integrals(w.volume, w.voxel_mm, w.offset_mm, pick, dist[32:96:4], side, side, reso)
+        dt = time.perf_counter() - t0
+        n_cpu = len(pick) * 16 * side * side
+        res["radon3d"]["cpu_baseline"] = {
+            "value": n_cpu / dt / 1e9, "unit": "Gsamples/s (template)", "cores": 1, "kind": "port",
+            "sample": f"ref32 oracle plane integrals (single thread), 8 orientations x 16 distances "
+                      f"of the same workload, {n_cpu} template samples in {dt:.2f} s"}
+    # slicer: 256 planes through the volume
+    rng = np.random.default_rng(7)
+    sl = ctl.VolumeSlicer(rt._vol)
+    sw, sh = sl.sliceDimensions()
+    nrm = rng.normal(size=(256, 3))
+    nrm = np.ascontiguousarray(nrm / np.linalg.norm(nrm, axis=1, keepdims=True))
+    dd = rng.uniform(-40, 40, 256)
+    sout = torch.empty(256 * sw * sh, dtype=torch.float32, device="cuda")
+
+    def slices():
+        _lib.check(L.ctlproj_slice_device(rt._vol.h, _lib.dptr(nrm), _lib.dptr(dd), 256, sw, sh,
+                                          sl.sliceResolution(), C.c_void_p(sout.data_ptr()), sout.numel()))
+
+    slices()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        slices()
+    swall = (time.perf_counter() - t0) / 3
+    res["slicer"] = dict(planes=256, width=sw, height=sh, ms=swall * 1e3,
+                         gsamples_s=256 * sw * sh / swall / 1e9,
+                         write_GBps=256 * sw * sh * 4 / swall / 1e9)
+    # resampler: 16 Mi points inside the ranges
+    rsm = ctl.VolumeResampler(rt._vol)
+    lo, hi = rsm._ranges[0::2], rsm._ranges[1::2]
+    n = 1 << 24
+    pts = torch.rand(n, 3, device="cuda") * torch.tensor(hi - lo, device="cuda") + torch.tensor(lo, device="cuda")
+    rout = torch.empty(n, dtype=torch.float32, device="cuda")
+    rsm.sample_device(pts.data_ptr(), n, rout.data_ptr())
+    t0 = time.perf_counter()
+    for _ in range(3):
+        rsm.sample_device(pts.data_ptr(), n, rout.data_ptr())
+    rwall = (time.perf_counter() - t0) / 3
+    res["resampler"] = dict(points=n, ms=rwall * 1e3, gsamples_s=n / rwall / 1e9,
+                            stream_GBps=n * 16 / rwall / 1e9)
+    log(f"processing: slicer {json.dumps(res['slicer'])} resampler {json.dumps(res['resampler'])}")
+    return res
+
+
 def cpu_baseline(w, oracle):
     threads = min(16, os.cpu_count() or 1)
     sub = list(range(0, 360, 45))  # 8 evenly strided views
@@ -304,6 +399,8 @@ def main() -> int:
                     help="real workload with the gathers from HBM through L1 (no LDS staging)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="with --only-real: one timed projection per variant and nothing else (profiling)")
+    ap.add_argument("--only-processing", action="store_true",
+                    help="only the volume processing leg (3D Radon, slicer, resampler; profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (calibrates the FETCH_SIZE / "
                          "WRITE_SIZE counters behind roofline.traffic) and exit")
@@ -331,12 +428,17 @@ def main() -> int:
     if not (args.no_cpu and args.no_errors):
         import oracle
 
+    if args.only_processing:
+        res = bench_processing(args, torch, W.real(views=1), ctl, oracle)
+        print(json.dumps({"processing": res}), flush=True)
+        return 0
     w, real = bench_real(args, torch, W, ctl, oracle)
     extra = {}
     if not args.only_real:
         extra["helical"] = bench_helical(args, torch, W, ctl, oracle)
         extra["batched"] = bench_batched(args, torch, W, ctl)
         extra["served"] = bench_served(args, torch, W, ctl)
+        extra["processing"] = bench_processing(args, torch, w, ctl, oracle)
     cpu = None if args.no_cpu else cpu_baseline(w, oracle)
     if cpu:
         log(f"cpu baseline: {cpu}")

@@ -216,12 +216,13 @@ static_assert(sizeof(PlaneMap) == 64, "PlaneMap is one 64-byte record");
 constexpr int kPlaneDists = 8;
 
 // One launch of plane integrals: n_orient orientations x n_dist distances over a width x height
-// template plane cut into 8x8 pixel tiles (tiles_x per row, n_tiles in all), each (orientation,
-// distance chunk) split into `splits` tile ranges.
+// template plane cut into tiles_x x tiles_y tiles of 8x8 pixels, visited in groups of 4x4 tiles
+// (n_tiles = 16 * groups, counting the unused places of partial groups); each (orientation,
+// distance chunk) is split into `splits` ranges of that order.
 struct PlaneJob {
     uint32_t n_orient, n_dist;
     uint32_t width, height;
-    uint32_t tiles_x, n_tiles;
+    uint32_t tiles_x, tiles_y, n_tiles;
     uint32_t splits;
 };
 

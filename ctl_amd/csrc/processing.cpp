@@ -173,8 +173,8 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
     ctlk::PlaneJob job{};
     job.n_dist = n_dist;
     job.width = job.height = side;
-    job.tiles_x = (side + 7) / 8;
-    job.n_tiles = job.tiles_x * job.tiles_x;
+    job.tiles_x = job.tiles_y = (side + 7) / 8;
+    job.n_tiles = ((job.tiles_x + 3) / 4) * ((job.tiles_y + 3) / 4) * 16;
     const uint64_t chunks = (uint64_t(n_dist) + ctlk::kPlaneDists - 1) / ctlk::kPlaneDists;
     // Orientations per launch: enough workgroups to fill the device several times over, each
     // split of the template plane at least 16 tiles, the partial sums within 64 Mi floats.

@@ -104,8 +104,15 @@ __global__ __launch_bounds__(kBlock) void sample_kernel(VolumeView vol, Ranges r
                                  r.scale[2] * (p[2] - r.lo[2]) + 0.5f);
 }
 
-// planeIntegral.cl: u = H (x, y, 0, 1)^T + d * shift for pixel (x, y) and distance d.
+// planeIntegral.cl: u = H (x, y, 0, 1)^T + d * shift for pixel (x, y) and distance d, read at the
+// texel position q = u + 1/2 (the 1/2 is folded into H's last column).
 // Logical workgroup b = (distance chunk, orientation, split), orientation fastest after the split.
+// Every lane reads the chunk's distances of its pixel back to back (independent gathers in
+// flight). The four wavefronts take the tiles of a row of a 4x4 tile group (32x32 pixels)
+// together and the group's rows one after the other, so tiles sharing texel rows are read close
+// together in time. A wavefront skips a tile when the box spanned by its corner pixels and the
+// chunk's distance range misses the volume's support on some axis (with half a texel to spare):
+// every sample of such a tile reads the zero border.
 template <Storage ST, Layout LY>
 __global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const PlaneMap* __restrict__ maps,
                                                        const float* __restrict__ dist, PlaneJob job,
@@ -115,47 +122,75 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const Pla
     const uint32_t split = b % job.splits, rest = b / job.splits;
     const uint32_t o = rest % job.n_orient, d0 = (rest / job.n_orient) * kPlaneDists;
     const uint32_t nd = min(uint32_t(kPlaneDists), job.n_dist - d0);
-    const uint32_t t0 = uint32_t(uint64_t(job.n_tiles) * split / job.splits);
-    const uint32_t t1 = uint32_t(uint64_t(job.n_tiles) * (split + 1) / job.splits);
+    const uint32_t v0 = uint32_t(uint64_t(job.n_tiles) * split / job.splits);
+    const uint32_t v1 = uint32_t(uint64_t(job.n_tiles) * (split + 1) / job.splits);
+    const uint32_t groups_x = (job.tiles_x + 3) / 4;
     const PlaneMap& m = maps[o];
-    const float h00 = m.h[0], h01 = m.h[1], h03 = m.h[3];
-    const float h10 = m.h[4], h11 = m.h[5], h13 = m.h[7];
-    const float h20 = m.h[8], h21 = m.h[9], h23 = m.h[11];
-    const float s0 = m.c[0], s1 = m.c[1], s2 = m.c[2];
+    float h[3][3], s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        h[k][0] = m.h[4 * k];
+        h[k][1] = m.h[4 * k + 1];
+        h[k][2] = m.h[4 * k + 3] + 0.5f;
+        s[k] = m.c[k];
+    }
     float dd[kPlaneDists], acc[kPlaneDists];
+    float dlo = dist[d0], dhi = dlo;
 #pragma unroll
     for (int j = 0; j < kPlaneDists; ++j) {
-        dd[j] = uint32_t(j) < nd ? dist[d0 + j] : 0.0f;
+        dd[j] = uint32_t(j) < nd ? dist[d0 + j] : dlo;
+        dlo = fminf(dlo, dd[j]);
+        dhi = fmaxf(dhi, dd[j]);
         acc[j] = 0.0f;
     }
     const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
     const Pitch p = pitchOf(vol);
     const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    // constant part and distance term's range of each texel coordinate
+    float elo[3], ehi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        elo[k] = h[k][2] + fminf(dlo * s[k], dhi * s[k]);
+        ehi[k] = h[k][2] + fmaxf(dlo * s[k], dhi * s[k]);
+    }
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t t = t0 + wave; t < t1; t += kBlock / 64) {
-        const uint32_t x = (t % job.tiles_x) * 8 + (lane & 7), y = (t / job.tiles_x) * 8 + (lane >> 3);
+    for (uint32_t v = v0 + wave; v < v1; v += kBlock / 64) {
+        const uint32_t g = v >> 4, w = v & 15;
+        const uint32_t tx = (g % groups_x) * 4 + (w & 3), ty = (g / groups_x) * 4 + (w >> 2);
+        if (tx >= job.tiles_x || ty >= job.tiles_y) continue;
+        const float x0 = float(tx * 8), x1 = float(tx * 8 + 7), y0 = float(ty * 8), y1 = float(ty * 8 + 7);
+        bool miss = false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float ax = h[k][0] * x0, bx = h[k][0] * x1, ay = h[k][1] * y0, by = h[k][1] * y1;
+            const float lo = elo[k] + fminf(ax, bx) + fminf(ay, by);
+            const float hi = ehi[k] + fmaxf(ax, bx) + fmaxf(ay, by);
+            miss = miss || hi < -0.5f || lo >= nhi[k] + 0.5f;
+        }
+        if (miss) continue; // uniform: the tile's values are the same for every lane
+        const uint32_t x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
         if (x >= job.width || y >= job.height) continue;
         const float fx = float(x), fy = float(y);
-        const float b0 = fmaf(h00, fx, fmaf(h01, fy, h03));
-        const float b1 = fmaf(h10, fx, fmaf(h11, fy, h13));
-        const float b2 = fmaf(h20, fx, fmaf(h21, fy, h23));
+        const float b0 = fmaf(h[0][0], fx, fmaf(h[0][1], fy, h[0][2]));
+        const float b1 = fmaf(h[1][0], fx, fmaf(h[1][1], fy, h[1][2]));
+        const float b2 = fmaf(h[2][0], fx, fmaf(h[2][1], fy, h[2][2]));
 #pragma unroll
         for (int j = 0; j < kPlaneDists; ++j)
             if (uint32_t(j) < nd)
-                acc[j] += sampleImage<ST, LY>(rs, p, nhi, fmaf(dd[j], s0, b0), fmaf(dd[j], s1, b1),
-                                              fmaf(dd[j], s2, b2));
+                acc[j] += sampleTexel<ST, LY>(rs, p, nhi, fmaf(dd[j], s[0], b0), fmaf(dd[j], s[1], b1),
+                                              fmaf(dd[j], s[2], b2));
     }
     __shared__ float red[kBlock / 64][kPlaneDists];
 #pragma unroll
     for (int j = 0; j < kPlaneDists; ++j) {
-        const float v = waveSum(acc[j]);
-        if (lane == 0) red[wave][j] = v;
+        const float r = waveSum(acc[j]);
+        if (lane == 0) red[wave][j] = r;
     }
     __syncthreads();
     if (threadIdx.x < nd) {
         const uint32_t j = threadIdx.x;
-        const float s = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
-        partial[(size_t(o) * job.n_dist + d0 + j) * job.splits + split] = s;
+        const float r = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+        partial[(size_t(o) * job.n_dist + d0 + j) * job.splits + split] = r;
     }
 }
 
@@ -246,7 +281,7 @@ hipError_t launchPlaneIntegrals(const VolumeView& vol, const PlaneMap* d_maps, c
                                 uint32_t out_stride, hipStream_t s)
 {
     if (job.n_orient == 0 || job.n_dist == 0) return hipSuccess;
-    if (job.splits == 0 || job.tiles_x == 0 || job.n_tiles == 0) return hipErrorInvalidValue;
+    if (job.splits == 0 || job.tiles_x == 0 || job.tiles_y == 0 || job.n_tiles == 0) return hipErrorInvalidValue;
     const uint64_t chunks = (uint64_t(job.n_dist) + kPlaneDists - 1) / kPlaneDists;
     const uint64_t blocks = chunks * job.n_orient * job.splits;
     if (blocks > 0x7fffffffull || uint64_t(job.n_orient) * job.n_dist > 0xffffffffull - kBlock)

@@ -152,16 +152,23 @@ __device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& 
 }
 
 // OpenCL linear image read at unnormalised coordinate u (the reference's read_imagef with
-// CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP): zero wherever all eight taps are border texels, that is
-// unless 0 <= u + 1/2 < n + 1 on every axis (nhi = n + 1). NaN coordinates read zero.
+// CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP) = sampleTexel at q = u + 1/2: zero wherever all eight
+// taps are border texels, that is unless 0 <= q < n + 1 on every axis (nhi = n + 1). NaN
+// coordinates read zero.
+template <Storage ST, Layout LY>
+__device__ __forceinline__ float sampleTexel(__amdgpu_buffer_rsrc_t rs, const Pitch& p, const float nhi[3],
+                                             float qx, float qy, float qz)
+{
+    if (!(qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2]))
+        return 0.0f;
+    return trilinear<ST, LY>(rs, p, qx, qy, qz);
+}
+
 template <Storage ST, Layout LY>
 __device__ __forceinline__ float sampleImage(__amdgpu_buffer_rsrc_t rs, const Pitch& p, const float nhi[3],
                                              float ux, float uy, float uz)
 {
-    const float qx = ux + 0.5f, qy = uy + 0.5f, qz = uz + 0.5f;
-    if (!(qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2]))
-        return 0.0f;
-    return trilinear<ST, LY>(rs, p, qx, qy, qz);
+    return sampleTexel<ST, LY>(rs, p, nhi, ux + 0.5f, uy + 0.5f, uz + 0.5f);
 }
 
 } // namespace

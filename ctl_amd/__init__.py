@@ -8,3 +8,5 @@ from .projector import (  # noqa: F401
     TubularGantry, VolumeData, VoxelVolume, WobbleTrajectory, ellipsoid_phantom,
 )
 from .processing import RadonTransform3D, SamplingRange, VolumeResampler, VolumeSlicer  # noqa: F401
+from .extensions import (DynamicProjectorExtension, LinearDynamicVolume, SpectralEffectsExtension,  # noqa: F401
+                         SpectralInformation, SpectralVolume)

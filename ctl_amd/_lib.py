@@ -154,6 +154,8 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_nrrd_write": (_i, [C.c_char_p, _vp, _i, _u32, C.POINTER(C.c_uint64), C.c_char_p, _f64p, _f64p,
                                 C.c_char_p]),
     "ctlproj_diag_gather_probe": (_i, [_vp, _vp, _i, _vp, _sz, _f32p, _f64p]),
+    "ctlproj_spectral_extinction": (_i, [_vp, _vp, _u32, _u32, _sz, _f32p, _f64p, _f32p, _u32, _f64p, _vp]),
+    "ctlproj_combine_views": (_i, [_vp, _vp, _f32p, _vp, _f32p, _u32, _sz, _vp]),
     "ctlproj_slicer_default_side": (_i, [_u32p, _u32p]),
     "ctlproj_slice": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _f32p, _sz]),
     "ctlproj_slice_device": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _vp, _sz]),

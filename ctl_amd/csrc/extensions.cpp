@@ -1,0 +1,94 @@
+// Device-side parts of the projector extensions that call the projection many times
+// (modules/src/projectors/spectraleffectsextension.cpp, dynamicprojectorextension.cpp): the
+// projections stay in device memory and are combined there by one kernel.
+#include "ctlproj.h"
+#include "runtime.h"
+
+#include <cmath>
+#include <vector>
+
+using namespace ctlrt;
+
+namespace {
+
+ctlproj_status begin(ctlproj_context* ctx)
+{
+    if (!ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "context is NULL");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hipFail(e, "hipSetDevice");
+    if (injected(ctx)) return CTLPROJ_ERR_INJECTED;
+    return CTLPROJ_OK;
+}
+
+ctlproj_status finish(hipError_t e, hipStream_t s, const char* what)
+{
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, what);
+}
+
+} // namespace
+
+ctlproj_status ctlproj_spectral_extinction(ctlproj_context* ctx, const float* device_material_projs,
+                                           uint32_t n_materials, uint32_t views, size_t pixels_per_view,
+                                           const float* mu, const double* intensity, const float* response,
+                                           uint32_t n_bins, const double* total_intensity, float* device_out)
+{
+    ctlproj_status st = begin(ctx);
+    if (st != CTLPROJ_OK) return st;
+    if (n_materials == 0 || n_materials > ctlk::kMaxMaterials)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "1 to 8 materials are supported");
+    if (views == 0 || pixels_per_view == 0) return CTLPROJ_OK;
+    if (!device_material_projs || !device_out || !total_intensity || (n_bins && (!mu || !intensity)))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    // bins whose intensities sum to zero over the views are skipped (spectraleffectsextension.cpp:
+    // 337-341, qFuzzyIsNull); intensities and I0 enter the transforms as float (singleviewdata.cpp:
+    // 137, 152)
+    std::vector<float> m, in, r, tot(views);
+    uint32_t used = 0;
+    for (uint32_t b = 0; b < n_bins; ++b) {
+        double sum = 0.0;
+        for (uint32_t v = 0; v < views; ++v) sum += intensity[size_t(b) * views + v];
+        if (std::fabs(sum) <= 1e-12) continue;
+        for (uint32_t k = 0; k < n_materials; ++k) m.push_back(mu[size_t(b) * n_materials + k]);
+        for (uint32_t v = 0; v < views; ++v) in.push_back(float(intensity[size_t(b) * views + v]));
+        r.push_back(response ? response[b] : 1.0f);
+        ++used;
+    }
+    for (uint32_t v = 0; v < views; ++v) tot[v] = float(total_intensity[v]);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    DeviceBuffer d_mu, d_in, d_r, d_tot;
+    if ((st = deviceUpload(d_mu, m.data(), m.size() * sizeof(float), s, "spectral constants")) != CTLPROJ_OK ||
+        (st = deviceUpload(d_in, in.data(), in.size() * sizeof(float), s, "spectral constants")) != CTLPROJ_OK ||
+        (st = deviceUpload(d_r, r.data(), r.size() * sizeof(float), s, "spectral constants")) != CTLPROJ_OK ||
+        (st = deviceUpload(d_tot, tot.data(), tot.size() * sizeof(float), s, "spectral constants")) != CTLPROJ_OK)
+        return st;
+    // the material projections may have been written on other streams
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = ctlk::launchSpectral(device_material_projs, n_materials, views, pixels_per_view, d_mu.as<float>(),
+                                 d_in.as<float>(), d_r.as<float>(), used, d_tot.as<float>(), device_out, s);
+    return finish(e, s, "spectral kernel");
+}
+
+ctlproj_status ctlproj_combine_views(ctlproj_context* ctx, const float* device_a, const float* coeff_a,
+                                     const float* device_b, const float* coeff_b, uint32_t views,
+                                     size_t pixels_per_view, float* device_out)
+{
+    ctlproj_status st = begin(ctx);
+    if (st != CTLPROJ_OK) return st;
+    if (views == 0 || pixels_per_view == 0) return CTLPROJ_OK;
+    if (!device_a || !coeff_a || !device_out || (device_b && !coeff_b))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipStream_t s = ctx->stream;
+    DeviceBuffer d_ca, d_cb;
+    if ((st = deviceUpload(d_ca, coeff_a, views * sizeof(float), s, "view coefficients")) != CTLPROJ_OK) return st;
+    if (device_b && (st = deviceUpload(d_cb, coeff_b, views * sizeof(float), s, "view coefficients")) != CTLPROJ_OK)
+        return st;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess)
+        e = ctlk::launchCombineViews(device_a, d_ca.as<float>(), device_b, device_b ? d_cb.as<float>() : nullptr,
+                                     views, pixels_per_view, device_out, s);
+    return finish(e, s, "combine kernel");
+}

@@ -198,6 +198,17 @@ hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, f
 // Mean of k contiguous projections of n values (intensity != 0: -ln of the mean of exp(-value)).
 hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity, float* d_out, hipStream_t s);
 
+// Spectral extinction of views x n pixels from n_mat (<= kMaxMaterials) material projections
+// stored one after the other ([m][v][pixel]); mu [bins][n_mat], inten [bins][views], resp [bins],
+// total [views].
+constexpr uint32_t kMaxMaterials = 8;
+hipError_t launchSpectral(const float* d_proj, uint32_t n_mat, uint32_t views, size_t n, const float* d_mu,
+                          const float* d_inten, const float* d_resp, uint32_t n_bins, const float* d_total,
+                          float* d_out, hipStream_t s);
+// out[v] = ca[v] a[v] + cb[v] b[v] per view of n values (d_b and d_cb may be NULL).
+hipError_t launchCombineViews(const float* d_a, const float* d_ca, const float* d_b, const float* d_cb,
+                              uint32_t views, size_t n, float* d_out, hipStream_t s);
+
 bool validShape(const LaunchShape& s);
 
 // ---- volume processing (processing.hip) -----------------------------------------------------

@@ -745,6 +745,51 @@ __global__ __launch_bounds__(kBlock) void average_kernel(const float* __restrict
     }
 }
 
+// Spectral extinction from material projections (SpectralEffectsExtension::projectLinear,
+// spectraleffectsextension.cpp:232-300, singleBinIntensityLinear :330-352, and the FP32 intensity
+// transforms of singleviewdata.cpp:135-156): for a pixel of view v with material projections p_m,
+//   e_b = sum_m p_m mu[b][m],  s = sum_b (i[b][v] exp(-e_b)) r[b],  out = ln(I0[v] / s),
+// the sums in material and bin order. Workgroup b covers pixels [256 (b % pb), ...) of view b / pb.
+__global__ __launch_bounds__(kBlock) void spectral_kernel(const float* __restrict__ proj, uint32_t n_mat,
+                                                          uint32_t views, size_t n, uint32_t pb,
+                                                          const float* __restrict__ mu,
+                                                          const float* __restrict__ inten,
+                                                          const float* __restrict__ resp, uint32_t n_bins,
+                                                          const float* __restrict__ total, float* __restrict__ out)
+{
+    const uint32_t v = blockIdx.x / pb;
+    const size_t i = size_t(blockIdx.x % pb) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float p[kMaxMaterials];
+#pragma unroll
+    for (uint32_t m = 0; m < kMaxMaterials; ++m)
+        p[m] = m < n_mat ? proj[(size_t(m) * views + v) * n + i] : 0.0f;
+    float s = 0.0f;
+    for (uint32_t b = 0; b < n_bins; ++b) {
+        float e = 0.0f;
+#pragma unroll
+        for (uint32_t m = 0; m < kMaxMaterials; ++m)
+            if (m < n_mat) e += p[m] * mu[b * n_mat + m];
+        s += inten[size_t(b) * views + v] * expf(-e) * resp[b];
+    }
+    out[size_t(v) * n + i] = logf(total[v] / s);
+}
+
+// Per-view linear combination out[v] = ca[v] a[v] + cb[v] b[v] (b may be NULL) of projections of
+// n values per view (DynamicProjectorExtension on volumes linear in time).
+__global__ __launch_bounds__(kBlock) void combine_views_kernel(const float* __restrict__ a, const float* __restrict__ ca,
+                                                               const float* __restrict__ b, const float* __restrict__ cb,
+                                                               size_t n, uint32_t pb, float* __restrict__ out)
+{
+    const uint32_t v = blockIdx.x / pb;
+    const size_t i = size_t(blockIdx.x % pb) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const size_t k = size_t(v) * n + i;
+    float r = ca[v] * a[k];
+    if (b) r += cb[v] * b[k];
+    out[k] = r;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Gather-ceiling probes. Diagnostics that measure the ceiling of the gather path; no projection
 // runs through them.
@@ -1079,6 +1124,29 @@ hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity,
     if (k == 0 || n == 0) return hipSuccess;
     const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 65536);
     average_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_in, k, n, intensity, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchSpectral(const float* d_proj, uint32_t n_mat, uint32_t views, size_t n, const float* d_mu,
+                          const float* d_inten, const float* d_resp, uint32_t n_bins, const float* d_total,
+                          float* d_out, hipStream_t s)
+{
+    if (n_mat == 0 || n_mat > kMaxMaterials) return hipErrorInvalidValue;
+    if (views == 0 || n == 0) return hipSuccess;
+    const uint64_t pb = (uint64_t(n) + kBlock - 1) / kBlock, blocks = pb * views;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    spectral_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_proj, n_mat, views, n, uint32_t(pb), d_mu, d_inten,
+                                                        d_resp, n_bins, d_total, d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchCombineViews(const float* d_a, const float* d_ca, const float* d_b, const float* d_cb,
+                              uint32_t views, size_t n, float* d_out, hipStream_t s)
+{
+    if (views == 0 || n == 0) return hipSuccess;
+    const uint64_t pb = (uint64_t(n) + kBlock - 1) / kBlock, blocks = pb * views;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
+    combine_views_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_a, d_ca, d_b, d_cb, n, uint32_t(pb), d_out);
     return hipGetLastError();
 }
 

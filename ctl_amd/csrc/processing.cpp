@@ -16,31 +16,11 @@ namespace {
 
 constexpr uint32_t kPatch = 16; // radontransform3d.cpp:6
 
-// Device allocation released on scope exit (after the call has synchronised its stream).
-struct DeviceBuffer {
-    void* p = nullptr;
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer&) = delete;
-    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer()
-    {
-        if (p) (void)hipFree(p);
-    }
-    template <class T> T* as() const { return static_cast<T*>(p); }
-};
-
-ctlproj_status alloc(DeviceBuffer& b, size_t bytes, const char* what)
-{
-    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
-    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, what);
-}
+ctlproj_status alloc(DeviceBuffer& b, size_t bytes, const char* what) { return deviceAlloc(b, bytes, what); }
 
 ctlproj_status upload(DeviceBuffer& b, const void* host, size_t bytes, hipStream_t s, const char* what)
 {
-    ctlproj_status st = alloc(b, bytes, what);
-    if (st != CTLPROJ_OK || bytes == 0) return st;
-    hipError_t e = hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s);
-    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, what);
+    return deviceUpload(b, host, bytes, s, what);
 }
 
 ctlproj_status begin(const ctlproj_volume* vol)

@@ -73,6 +73,34 @@ struct EffectiveVolume {
 ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings& s,
                                hipStream_t stream, EffectiveVolume& ev);
 
+// Device allocation released on scope exit (callers synchronise their stream before that).
+struct DeviceBuffer {
+    void* p = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+inline ctlproj_status deviceAlloc(DeviceBuffer& b, size_t bytes, const char* what)
+{
+    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 1);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, what);
+}
+
+// Allocates b and copies `bytes` from host memory on stream s.
+inline ctlproj_status deviceUpload(DeviceBuffer& b, const void* host, size_t bytes, hipStream_t s, const char* what)
+{
+    ctlproj_status st = deviceAlloc(b, bytes, what);
+    if (st != CTLPROJ_OK || bytes == 0) return st;
+    hipError_t e = hipMemcpyAsync(b.p, host, bytes, hipMemcpyHostToDevice, s);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, what);
+}
+
 ctlproj_status validateSettings(const ctlproj_settings& s);
 // True if projections with these settings and this volume storage use the LDS-staged kernel.
 bool stagedWorkload(const ctlproj_settings& s, ctlk::Storage st, ctlk::Layout ly);

@@ -186,15 +186,34 @@ class AcquisitionSetup:
 
     def __init__(self, system: CTSystem, nbViews: int = 0):
         self.system = system
-        self._nb_views = int(nbViews)
+        self._nb_views = 0
+        self._time_stamps: list[float] = []
         self._protocol = None
         self._poses = None
+        self.setNbViews(nbViews)
 
     def setNbViews(self, n: int) -> None:
-        self._nb_views = int(n)
+        """Removes excess views, or appends views whose time stamps continue from the last view
+        with the last increment (1.0 with fewer than two views), acquisitionsetup.cpp:534-551."""
+        n = int(n)
+        ts = self._time_stamps
+        if n <= len(ts):
+            del ts[n:]
+        else:
+            last = ts[-1] if ts else -1.0
+            inc = ts[-1] - ts[-2] if len(ts) > 1 else 1.0
+            ts.extend(last + (k + 1) * inc for k in range(n - len(ts)))
+        self._nb_views = n
 
     def nbViews(self) -> int:
         return self._nb_views
+
+    def timeStamp(self, view: int) -> float:
+        """AcquisitionSetup::View::timeStamp (acquisitionsetup.cpp:36)."""
+        return self._time_stamps[view]
+
+    def setTimeStamp(self, view: int, t: float) -> None:
+        self._time_stamps[view] = float(t)
 
     def applyPreparationProtocol(self, protocol) -> None:
         self._protocol = protocol
@@ -204,7 +223,7 @@ class AcquisitionSetup:
         """Per-view C-arm source locations (prepare::CarmGantryParam::setLocation)."""
         self._poses = (np.ascontiguousarray(positions, np.float64),
                        np.ascontiguousarray(rotations, np.float64))
-        self._nb_views = len(self._poses[0])
+        self.setNbViews(len(self._poses[0]))
         self._protocol = None
 
 

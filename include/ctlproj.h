@@ -149,6 +149,29 @@ ctlproj_status ctlproj_project_composite_device(ctlproj_projector* p, const ctlp
 ctlproj_status ctlproj_average_projections(ctlproj_context* ctx, const float* device_in, uint32_t k,
                                            size_t n, int intensity_domain, float* device_out);
 
+/* SpectralEffectsExtension::projectLinear (spectraleffectsextension.cpp:232-300, 330-352): the
+ * extinction seen through an X-ray spectrum, from the projections of n_materials (1..8) material
+ * densities stored one after the other in device memory (n_materials blocks of views *
+ * pixels_per_view floats). For each pixel of view v:
+ *   out = ln( I0[v] / sum_b response[b] * intensity[b][v] * exp(-sum_m mu[b][m] * proj_m) )
+ * mu [n_bins][n_materials]: mass attenuation coefficients of each bin in the units of the density
+ * projections (the reference multiplies cm^2/g by 0.1 for g/cm^3 densities in mm);
+ * intensity [n_bins][views]: the bins' initial intensities per view; response [n_bins]: detector
+ * response per bin or NULL (no response model); total_intensity [views]: I0. Bins whose
+ * intensities sum to zero are skipped, as in the reference. Synchronous on the context's stream. */
+ctlproj_status ctlproj_spectral_extinction(ctlproj_context* ctx, const float* device_material_projs,
+                                           uint32_t n_materials, uint32_t views, size_t pixels_per_view,
+                                           const float* mu, const double* intensity, const float* response,
+                                           uint32_t n_bins, const double* total_intensity, float* device_out);
+/* Per-view linear combination of two projections in device memory:
+ * out[v] = coeff_a[v] * a[v] + coeff_b[v] * b[v] (device_b NULL: the second term is absent).
+ * DynamicProjectorExtension (dynamicprojectorextension.cpp:39-66) on volumes that are linear in
+ * time (LinearDynamicVolume, lineardynamicvolume.cpp:194-198: slope * t + offset) is
+ * t_v * P_v(slope) + P_v(offset). Synchronous on the context's stream. */
+ctlproj_status ctlproj_combine_views(ctlproj_context* ctx, const float* device_a, const float* coeff_a,
+                                     const float* device_b, const float* coeff_b, uint32_t views,
+                                     size_t pixels_per_view, float* device_out);
+
 ctlproj_status ctlproj_projector_synchronize(ctlproj_projector* p);
 
 /* Duration of the last ray-casting kernel on the projector's stream (HIP events), in ms. */

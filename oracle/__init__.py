@@ -219,6 +219,28 @@ def project(volume, voxel_mm, offset_mm, pmats, channels, rows, ray_sampling=0.3
     return out, n.value
 
 
+def spectral_extinction(material_projs, mu, intensity, response=None, total_intensity=None):
+    """SpectralEffectsExtension::projectLinear (spectraleffectsextension.cpp:232-300, 330-352) in
+    FP64: material_projs (n_mat, V, ...) projections of the densities, mu (n_bins, n_mat),
+    intensity (n_bins, V); per pixel ln(I0_v / sum_b r_b i_bv exp(-sum_m mu_bm p_m)). Bins whose
+    intensities sum to zero over the views are skipped (:337-341); I0 defaults to the sum over
+    bins."""
+    p = np.asarray(material_projs, np.float64)
+    mu = np.asarray(mu, np.float64)
+    inten = np.asarray(intensity, np.float64)
+    n_bins, V = inten.shape
+    r = np.ones(n_bins) if response is None else np.asarray(response, np.float64)
+    i0 = inten.sum(axis=0) if total_intensity is None else np.asarray(total_intensity, np.float64)
+    shape = (V,) + (1,) * (p.ndim - 2)
+    s = np.zeros(p.shape[1:])
+    for b in range(n_bins):
+        if abs(inten[b].sum()) <= 1e-12:
+            continue
+        e = np.tensordot(mu[b], p, axes=(0, 0))
+        s += inten[b].reshape(shape) * np.exp(-e) * r[b]
+    return np.log(i0.reshape(shape) / s)
+
+
 def rel_l2(ref, x) -> float:
     """RelativeL2Norm (processing/errormetrics.cpp:69-85): ||ref - x|| / ||ref||, ref first."""
     r = np.asarray(ref, np.float64).ravel()

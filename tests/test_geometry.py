@@ -157,3 +157,16 @@ def test_source_shift_moves_the_source_in_its_own_frame():
     dec1 = GeometryDecoder((64, 32), pix).decodeFullGeometry(GeometryEncoder.encodeFullGeometry(setup, shift))
     assert np.allclose(dec1.source_positions, pos + rot @ shift, rtol=0, atol=1e-9)
     assert np.allclose(dec1.module_positions, dec0.module_positions, rtol=0, atol=1e-9)
+
+
+def test_view_time_stamps_follow_the_reference_rule():
+    """acquisitionsetup.cpp:534-551: new views continue from the last time stamp with the last
+    increment (1.0 with fewer than two views); fewer views drop the excess."""
+    from ctl_amd import AcquisitionSetup, CarmGantry, CTSystem, FlatPanelDetector
+    s = AcquisitionSetup(CTSystem(CarmGantry(1200.0), FlatPanelDetector((8, 8), (1.0, 1.0))), 3)
+    assert [s.timeStamp(v) for v in range(3)] == [0.0, 1.0, 2.0]
+    s.setTimeStamp(2, 5.0)
+    s.setNbViews(5)
+    assert [s.timeStamp(v) for v in range(5)] == [0.0, 1.0, 5.0, 9.0, 13.0]
+    s.setNbViews(2)
+    assert s.nbViews() == 2 and [s.timeStamp(v) for v in range(2)] == [0.0, 1.0]

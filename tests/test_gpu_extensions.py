@@ -1,14 +1,17 @@
-"""Multi-projection callers on resident volumes (SURVEY rank-2 callers of the projector):
-projectComposite (abstractprojector.h:99-112) accumulated on the device, and
-ArealFocalSpotExtension (arealfocalspotextension.cpp:82-166, 257-292) as one launch over all
-focal-spot samples plus a device average. Checked against per-volume projections and against the
-oracle projected at each shifted geometry and averaged in FP64."""
+"""Multi-projection callers on resident volumes: projectComposite (abstractprojector.h:99-112)
+accumulated on the device; ArealFocalSpotExtension (arealfocalspotextension.cpp:82-166, 257-292)
+as one launch over all focal-spot samples plus a device average; SpectralEffectsExtension's
+linear path (spectraleffectsextension.cpp:232-352) as one projection per material plus a device
+sum over energy bins; DynamicProjectorExtension (dynamicprojectorextension.cpp:39-66) on linear
+and general dynamic volumes. Checked against per-volume projections and against the oracle."""
 import numpy as np
 import pytest
 
 import oracle
-from ctl_amd import (AcquisitionSetup, ArealFocalSpotExtension, CarmGantry, CTSystem, FlatPanelDetector,
-                     GeometryEncoder, RayCasterProjector, ResidentVolume, ShortScanTrajectory, VoxelVolume)
+from ctl_amd import (AcquisitionSetup, ArealFocalSpotExtension, CarmGantry, CTSystem, DynamicProjectorExtension,
+                     FlatPanelDetector, GeometryEncoder, LinearDynamicVolume, RayCasterProjector, ResidentVolume,
+                     ShortScanTrajectory, SpectralEffectsExtension, SpectralInformation, SpectralVolume,
+                     VoxelVolume)
 from ctl_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -60,3 +63,89 @@ def test_areal_focal_spot_against_oracle(low_extinction):
     p0.settings().raySampling = 0.1
     p0.configure(setup)
     assert oracle.rel_l2(p0.project(VoxelVolume(w.volume, (1.0, 1.0, 1.0))).data, got) > 1e-4
+
+
+def test_spectral_effects_against_oracle():
+    """SpectralEffectsExtension's linear path: one projection per material density and the bins
+    summed on the device, against ref32 projections combined in FP64 (oracle.spectral_extinction)."""
+    w = W.small(views=4)
+    setup = small_setup()
+    d1 = np.ascontiguousarray(w.volume / w.volume.max() * 1.2, np.float32)           # water-like
+    d2 = np.ascontiguousarray(w.volume[::-1, :, ::-1] / w.volume.max() * 0.8, np.float32)  # bone-like
+    energies = [30.0, 50.0, 70.0, 90.0, 110.0]
+    rng = np.random.default_rng(21)
+    inten = rng.uniform(0.5, 2.0, (5, 4))
+    inten[3] = 0.0  # a bin without photons is skipped
+    ma1 = [0.375, 0.227, 0.193, 0.171, 0.160]  # cm^2/g
+    ma2 = [1.33, 0.425, 0.253, 0.196, 0.172]
+    resp = [0.7, 0.9, 1.0, 1.0, 0.95]
+    nested = RayCasterProjector()
+    nested.settings().raySampling = 0.1
+    ext = SpectralEffectsExtension(nested, SpectralInformation(energies, inten))
+    ext.setDetectorResponse(resp)
+    ext.configure(setup)
+    got = ext.projectComposite([SpectralVolume(VoxelVolume(d1, (1.0, 1.0, 1.0)), ma1),
+                                SpectralVolume(VoxelVolume(d2, (1.0, 1.0, 1.0)), ma2)]).data
+    pm = GeometryEncoder.encodeFullGeometry(setup)
+    p1, _ = oracle.project(d1, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), pm, 256, 128, 0.1)
+    p2, _ = oracle.project(d2, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), pm, 256, 128, 0.1)
+    mu = 0.1 * np.array([ma1, ma2]).T
+    ref = oracle.spectral_extinction(np.stack([p1, p2]), mu, inten, resp)
+    e = oracle.rel_l2(ref, got)
+    print(f"spectral effects: rel L2 vs ref32 projections + FP64 spectrum {e:.3e}, max extinction {ref.max():.2f}")
+    assert got.shape == (4, 1, 128, 256) and e <= 1e-4
+    assert not ext.isLinear()
+
+
+def test_dynamic_linear_volume_against_oracle():
+    """DynamicProjectorExtension on a LinearDynamicVolume: t_v * P_v(slope) + P_v(offset) on the
+    device against ref32 projections of slope * t_v + offset, one view at a time."""
+    w = W.small(views=4)
+    setup = small_setup()
+    for v, t in enumerate([0.0, 12.5, 30.0, 47.0]):
+        setup.setTimeStamp(v, t)
+    slope = VoxelVolume(np.ascontiguousarray(w.volume * 0.02, np.float32), (1.0, 1.0, 1.0))
+    offset = VoxelVolume(np.ascontiguousarray(w.volume[:, ::-1, :], np.float32), (1.0, 1.0, 1.0))
+    dyn = LinearDynamicVolume(slope, offset)
+    nested = RayCasterProjector()
+    nested.settings().raySampling = 0.1
+    ext = DynamicProjectorExtension(nested)
+    ext.configure(setup)
+    got = ext.project(dyn).data
+    pm = GeometryEncoder.encodeFullGeometry(setup)
+    ref = np.concatenate([oracle.project(dyn.at(setup.timeStamp(v)).data, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0),
+                                         pm[v:v + 1], 256, 128, 0.1)[0] for v in range(4)])
+    e = oracle.rel_l2(ref, got)
+    print(f"dynamic (linear): rel L2 vs ref32 {e:.3e}")
+    assert got.shape == (4, 1, 128, 256) and e <= 1e-4
+    # the views differ by more than the tolerance: the time stamps matter
+    assert oracle.rel_l2(ref[3], ref[0]) > 1e-2
+
+
+def test_dynamic_general_volume_view_by_view():
+    """Any volume with setTime(): one projection per view at that view's time stamp."""
+    w = W.small(views=4)
+    setup = small_setup()
+
+    class Pulsing:
+        def __init__(self, base):
+            self.base = base
+            self.setTime(0.0)
+
+        def setTime(self, t):
+            self.volume = VoxelVolume(np.ascontiguousarray(self.base * np.float32(1.0 + 0.5 * np.sin(t))),
+                                      (1.0, 1.0, 1.0))
+
+    nested = RayCasterProjector()
+    nested.settings().raySampling = 0.1
+    ext = DynamicProjectorExtension(nested)
+    ext.configure(setup)  # default time stamps 0, 1, 2, 3
+    got = ext.project(Pulsing(w.volume)).data
+    pm = GeometryEncoder.encodeFullGeometry(setup)
+    ref = np.concatenate([oracle.project(w.volume * np.float32(1.0 + 0.5 * np.sin(float(v))), (1.0, 1.0, 1.0),
+                                         (0.0, 0.0, 0.0), pm[v:v + 1], 256, 128, 0.1)[0] for v in range(4)])
+    assert oracle.rel_l2(ref, got) <= 1e-4
+    # a static volume goes straight to the nested projector
+    static = ext.project(VoxelVolume(w.volume, (1.0, 1.0, 1.0))).data
+    full, _ = oracle.project(w.volume, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), pm, 256, 128, 0.1)
+    assert oracle.rel_l2(full, static) <= 1e-4

@@ -66,12 +66,13 @@ def build(verbose: bool = False) -> Path:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
           "-pthread"])
     os.replace(tmp, lib)
-    # the C++ mirror's demo/test program, linked against the library
-    demo = ROOT / "tests" / "cpp" / "projector_demo.cpp"
-    if demo.exists():
-        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(demo), "-o",
-              str(LIB / "projector_demo"), f"-L{LIB}", "-lctlproj", "-Wl,-rpath,$ORIGIN",
-              f"-L{ROCM / 'lib'}", f"-Wl,-rpath,{ROCM / 'lib'}", "-lamdhip64"])
+    # the C++ mirror's demo/test programs, linked against the library
+    for name in ("projector_demo", "processing_demo"):
+        demo = ROOT / "tests" / "cpp" / f"{name}.cpp"
+        if demo.exists():
+            _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(demo), "-o",
+                  str(LIB / name), f"-L{LIB}", "-lctlproj", "-Wl,-rpath,$ORIGIN",
+                  f"-L{ROCM / 'lib'}", f"-Wl,-rpath,{ROCM / 'lib'}", "-lamdhip64"])
     if verbose:
         print(f"built {lib}")
     return lib

@@ -1,7 +1,8 @@
 // ctlproj.hpp -- header-only C++ mirror of CTL's projector interface over the C ABI (ctlproj.h).
 //
 // Mirrors CTL::OCL::RayCasterProjector (modules/src/projectors/raycasterprojector.h:20-58) and
-// AbstractProjector (modules/src/projectors/abstractprojector.h:65-112) without Qt:
+// AbstractProjector (modules/src/projectors/abstractprojector.h:65-112) without Qt, and the OCL
+// processing classes VolumeSlicer, VolumeResampler and RadonTransform3D on resident volumes:
 //   RayCasterProjector p;  p.settings().raySampling = 0.2f;
 //   p.configure(pmats, viewDim);            // FullGeometry as [views][modules][3][4] doubles
 //   ProjectionData d = p.project(volume);   // VoxelVolume<float>, x fastest
@@ -13,7 +14,9 @@
 
 #include "ctlproj.h"
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <functional>
 #include <memory>
 #include <stdexcept>
@@ -227,6 +230,130 @@ private:
     Settings settings_;
     ViewDimensions viewDim_;
     uint32_t nbViews_ = 0;
+};
+
+// Chunk2D<float> (img/chunk2d.h): width x height, x fastest.
+struct Chunk2D {
+    uint32_t width = 0, height = 0;
+    std::vector<float> data;
+    float operator()(uint32_t x, uint32_t y) const { return data[size_t(y) * width + x]; }
+};
+
+// Unit normal from polar coordinates, as volumeslicer.cpp:172-176 and radontransform3d.cpp:226-230.
+inline std::array<double, 3> planeNormal(double azimuth, double polar)
+{
+    return { { std::sin(polar) * std::cos(azimuth), std::sin(polar) * std::sin(azimuth), std::cos(polar) } };
+}
+
+// OCL::VolumeSlicer (processing/volumeslicer.h:37-60) on a resident volume.
+class VolumeSlicer {
+public:
+    VolumeSlicer(Context& ctx, const VoxelVolume& volume, ctlproj_dtype storage = CTLPROJ_F32)
+        : vol_(ctx, volume, storage)
+    {
+        uint32_t side = 0;
+        check(ctlproj_slicer_default_side(volume.dims.data(), &side));
+        width_ = height_ = side;
+        reso_ = std::min(volume.voxelSize[0], std::min(volume.voxelSize[1], volume.voxelSize[2]));
+    }
+    void setSliceDimensions(uint32_t width, uint32_t height) { width_ = width; height_ = height; }
+    void setSliceResolution(float pixelResolution) { reso_ = pixelResolution; }
+    float sliceResolution() const { return reso_; }
+    Chunk2D slice(const std::array<double, 3>& planeUnitNormal, double planeDistanceFromOrigin) const
+    {
+        Chunk2D ret;
+        ret.width = width_;
+        ret.height = height_;
+        ret.data.resize(size_t(width_) * height_);
+        check(ctlproj_slice(vol_.get(), planeUnitNormal.data(), &planeDistanceFromOrigin, 1, width_, height_, reso_,
+                            ret.data.data(), ret.data.size()));
+        return ret;
+    }
+    Chunk2D slice(double azimuth, double polar, double distance) const
+    {
+        return slice(planeNormal(azimuth, polar), distance);
+    }
+
+private:
+    ResidentVolume vol_;
+    uint32_t width_ = 0, height_ = 0;
+    float reso_ = 1.0f;
+};
+
+// OCL::VolumeResampler (processing/volumeresampler.h:11-56) on a resident volume. Ranges are
+// {lo1, hi1, lo2, hi2, lo3, hi3} (SamplingRange per axis).
+class VolumeResampler {
+public:
+    VolumeResampler(Context& ctx, const VoxelVolume& volume, ctlproj_dtype storage = CTLPROJ_F32)
+        : vol_(ctx, volume, storage)
+    {
+        check(ctlproj_resampler_default_ranges(vol_.get(), ranges_.data()));
+    }
+    void setSamplingRanges(const std::array<float, 6>& ranges) { ranges_ = ranges; }
+    const std::array<float, 6>& ranges() const { return ranges_; }
+    // resample: out[i + j*n1 + k*n1*n2] at (p1[i], p2[j], p3[k])
+    std::vector<float> resample(const std::vector<float>& p1, const std::vector<float>& p2,
+                                const std::vector<float>& p3) const
+    {
+        std::vector<float> out(p1.size() * p2.size() * p3.size());
+        check(ctlproj_resample(vol_.get(), ranges_.data(), p1.data(), uint32_t(p1.size()), p2.data(),
+                               uint32_t(p2.size()), p3.data(), uint32_t(p3.size()), out.data(), out.size()));
+        return out;
+    }
+    // sample: xyz holds 3 floats per point (Generic3DCoord)
+    std::vector<float> sample(const std::vector<float>& xyz) const
+    {
+        std::vector<float> out(xyz.size() / 3);
+        check(ctlproj_sample(vol_.get(), ranges_.data(), xyz.data(), out.size(), out.data()));
+        return out;
+    }
+
+private:
+    ResidentVolume vol_;
+    std::array<float, 6> ranges_{};
+};
+
+// OCL::RadonTransform3D (processing/radontransform3d.h:92-173) on a resident volume.
+class RadonTransform3D {
+public:
+    RadonTransform3D(Context& ctx, const VoxelVolume& volume, ctlproj_dtype storage = CTLPROJ_F32)
+        : vol_(ctx, volume, storage)
+    {
+        check(ctlproj_radon3d_default_slice(volume.dims.data(), volume.voxelSize.data(), &side_, &reso_));
+    }
+    void setSliceResolution(float pixelResolution)
+    {
+        check(ctlproj_radon3d_side_for_resolution(side_, reso_, pixelResolution, &side_));
+        reso_ = pixelResolution;
+    }
+    uint32_t sliceSide() const { return side_; }
+    float sliceResolution() const { return reso_; }
+    float planeIntegral(const std::array<double, 3>& planeUnitNormal, double planeDistanceFromOrigin) const
+    {
+        const float d = float(planeDistanceFromOrigin);
+        float r = 0.0f;
+        check(ctlproj_radon3d_plane_integrals(vol_.get(), planeUnitNormal.data(), 1, &d, 1, side_, reso_, &r, 1));
+        return r;
+    }
+    float planeIntegral(double azimuth, double polar, double distance) const
+    {
+        return planeIntegral(planeNormal(azimuth, polar), distance);
+    }
+    // sampleTransform(azimuths, polars, distances): out[a + nAzi*(p + nPol*d)]
+    std::vector<float> sampleTransform(const std::vector<float>& azimuth, const std::vector<float>& polar,
+                                       const std::vector<float>& distance) const
+    {
+        std::vector<float> out(azimuth.size() * polar.size() * distance.size());
+        check(ctlproj_radon3d_sample_transform(vol_.get(), azimuth.data(), uint32_t(azimuth.size()), polar.data(),
+                                               uint32_t(polar.size()), distance.data(), uint32_t(distance.size()),
+                                               side_, reso_, out.data(), out.size()));
+        return out;
+    }
+
+private:
+    ResidentVolume vol_;
+    uint32_t side_ = 0;
+    float reso_ = 1.0f;
 };
 
 } // namespace ctl_amd

@@ -252,3 +252,36 @@ def test_processing_error_paths():
     # empty requests succeed without touching the outputs
     assert L.ctlproj_slice(vol.h, _lib.dptr(n), _lib.dptr(d), 0, 4, 4, 1.0, None, 0) == _lib.OK
     assert L.ctlproj_radon3d_plane_integrals(vol.h, _lib.dptr(n), 1, _lib.fptr(dist), 0, 16, 1.0, None, 0) == _lib.OK
+
+
+def test_cpp_mirror_processing_program(tmp_path):
+    """The C++ mirrors (include/ctlproj.hpp: VolumeSlicer, VolumeResampler, RadonTransform3D),
+    driven like CTL code, against the oracle."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "ctl_amd" / "lib" / "processing_demo"
+    out = tmp_path / "proc.f32"
+    res = subprocess.run([str(exe), str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "slice=57x57 radon=64" in res.stdout  # ceil(1.4142 * 40) = 57; 57 -> 64
+    raw = np.fromfile(out, np.float32)
+    dims, voxel, offset = (40, 36, 32), (0.9, 1.1, 0.8), (1.5, -2.0, 0.7)
+    n_vol = 40 * 36 * 32
+    vol = raw[:n_vol].reshape(32, 36, 40)
+    sl = raw[n_vol:n_vol + 57 * 57].reshape(57, 57)
+    smp = raw[n_vol + 57 * 57:n_vol + 57 * 57 + 7]
+    tr = raw[n_vol + 57 * 57 + 7:].reshape(4, 2, 3)
+    n = np.array([np.sin(1.1) * np.cos(0.4), np.sin(1.1) * np.sin(0.4), np.cos(1.1)])
+    want = oracle.slice_plane(vol, voxel, offset, n, 2.5, 57, 57, 0.8)
+    assert np.abs(sl - want).max() <= 1e-4 * max(1.0, float(np.abs(vol).max()))
+    ranges = []
+    for k in range(3):
+        half = np.float32(0.5) * np.float32(voxel[k]) * np.float32(dims[k] - 1)
+        ranges += [np.float32(offset[k]) - half, np.float32(offset[k]) + half]
+    pts = np.array([0, 0, 0, 5.2, -3.1, 4, -17, 16, -11, 1.5, -2, 0.7, 20, 0, 0, -9.3, 7.7, 2.2, 3.3, -18, 11.9],
+                   np.float32).reshape(7, 3)
+    assert np.abs(smp - oracle.sample_points(vol, ranges, pts)).max() <= 1e-4 * max(1.0, float(np.abs(vol).max()))
+    normals = np.array([[np.sin(float(p)) * np.cos(float(a)), np.sin(float(p)) * np.sin(float(a)), np.cos(float(p))]
+                        for p in np.float32([0.5, 1.5]) for a in np.float32([0.0, 1.0, 2.0])])
+    r32 = oracle.plane_integrals(vol, voxel, offset, normals, np.float32([-6, -1, 0.5, 7]), 64, 64, 0.8)
+    assert np.abs(tr.reshape(4, 6) - r32).max() <= 2e-5 * np.abs(r32).max()

@@ -440,12 +440,14 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
         if (!pixel) ray.hit = false;
         tighten(ray, g.c, kc.blo, kc.nhi);
         const bool act = pixel && ray.i0 <= ray.i1;
-        // sample range of the workgroup
-        {
-            const int lo = waveReduce<false>(act ? ray.i0 : INT_MAX);
-            const int hi = waveReduce<true>(act ? ray.i1 : INT_MIN);
-            if (lane == 0) { red[1][wave][0] = lo; red[1][wave][1] = hi; }
-        }
+        // sample ranges, once per tile: the union [r_lo, r_hi] and the intersection [r_alo, r_ahi]
+        // of the wave's rays, and the workgroup's union. Clipped to a chunk, the union covers
+        // every live lane's samples and every live lane samples all of the intersection.
+        const int r_lo = waveReduce<false>(act ? ray.i0 : INT_MAX);
+        const int r_hi = waveReduce<true>(act ? ray.i1 : INT_MIN);
+        const int r_alo = waveReduce<true>(act ? ray.i0 : INT_MIN);
+        const int r_ahi = waveReduce<false>(act ? ray.i1 : INT_MAX);
+        if (lane == 0) { red[1][wave][0] = r_lo; red[1][wave][1] = r_hi; }
         __syncthreads();
         const int blo = min(min(red[1][0][0], red[1][1][0]), min(red[1][2][0], red[1][3][0]));
         const int bhi = max(max(red[1][0][1], red[1][1][1]), max(red[1][2][1], red[1][3][1]));
@@ -460,11 +462,11 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             const int e = act ? min(c1, ray.i1) : INT_MIN;
             const bool live = a <= e;
             // Texel box of this lane's taps in the chunk, [lo, hi] per axis (texels carry z+1, and
-            // y+1 for the FP16/BF16 quads, so only x and for FP32 y need one more), and the lane's
-            // sample range relative to c0. Reduced over the wave as packed 16-bit minima (a maximum
-            // m as the minimum of 0xffff - m): padded sizes stay below 65535 when a box is staged
-            // (kc.lds_texels > 0) and ranges below the chunk size. 0xffff marks "no live lane".
-            uint32_t pk[5];
+            // y+1 for the FP16/BF16 quads, so only x and for FP32 y need one more). Reduced over
+            // the wave as packed 16-bit minima (a maximum m as the minimum of 0xffff - m): padded
+            // sizes stay below 65535 when a box is staged (kc.lds_texels > 0). 0xffff marks "no
+            // live lane".
+            uint32_t pk[3];
             {
                 const int as = live ? a : c0, es = live ? e : c0; // in range for every lane
                 uint32_t lo[3], hi[3];
@@ -475,24 +477,17 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     lo[k] = uint32_t(int(fminf(qa, qe))) & 0xffffu;
                     hi[k] = 0xffffu - ((uint32_t(int(fmaxf(qa, qe))) + extra) & 0xffffu);
                 }
-                const uint32_t ra = uint32_t(as - c0), re = uint32_t(es - c0);
                 pk[0] = live ? lo[0] | lo[1] << 16 : ~0u;
                 pk[1] = live ? hi[0] | hi[1] << 16 : ~0u;
                 pk[2] = live ? lo[2] | hi[2] << 16 : ~0u;
-                pk[3] = live ? ra | (0xffffu - ra) << 16 : ~0u;
-                pk[4] = live ? re | (0xffffu - re) << 16 : ~0u;
             }
 #pragma unroll
-            for (int k = 0; k < 5; ++k) pk[k] = wavePkMin(pk[k]);
-            // the wave's sample range [wlo, whi] and the part [wa, we] all its live lanes share
-            int wlo = INT_MAX, whi = INT_MIN, wa = INT_MIN + 1, we = INT_MIN;
-            if ((pk[3] & 0xffffu) != 0xffffu) {
-                wlo = c0 + int(pk[3] & 0xffffu);
-                wa = c0 + int(0xffffu - (pk[3] >> 16));
-                we = c0 + int(pk[4] & 0xffffu);
-                whi = c0 + int(0xffffu - (pk[4] >> 16));
-                if (wa > we) { wa = whi + 1; we = whi; } // no shared range: all masked
-            }
+            for (int k = 0; k < 3; ++k) pk[k] = wavePkMin(pk[k]);
+            // the wave's sample range [wlo, whi] in this chunk and the part [wa, we] all its live
+            // lanes share (empty: all masked)
+            int wlo = max(c0, r_lo), whi = min(c1, r_hi), wa = max(c0, r_alo), we = min(c1, r_ahi);
+            if (wlo > whi) { wlo = INT_MAX; whi = INT_MIN; wa = INT_MIN + 1; we = INT_MIN; }
+            else if (wa > we) { wa = whi + 1; we = whi; }
             if (lane == 0)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) red[parity][wave][k] = int(pk[k]);

@@ -472,6 +472,7 @@ ctlproj_status ctlproj_volume_destroy(ctlproj_volume* vol)
     if (!vol) return CTLPROJ_OK;
     (void)hipSetDevice(vol->ctx->device);
     if (vol->d_data) (void)hipFree(vol->d_data);
+    if (vol->d_brick) (void)hipFree(vol->d_brick);
     delete vol;
     return CTLPROJ_OK;
 }

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -145,6 +146,27 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
         hipError_t e = ctlk::launchZero(d_out, size_t(n_orient) * n_dist, s);
         return finish(e, s, "zero fill");
     }
+    // the volume's brick copy (made once, kept with the volume); without it the kernel reads the
+    // resident texels. CTLPROJ_PLANE_READS=texels keeps the texel reads.
+    const float* brick = nullptr;
+    const char* env = std::getenv("CTLPROJ_PLANE_READS");
+    uint32_t nb[3];
+    size_t brick_bytes = 0;
+    if (!(env && std::strcmp(env, "texels") == 0) && ctlk::brickShape(vol->n, nb, &brick_bytes)) {
+        if (!vol->d_brick) {
+            void* p = nullptr;
+            hipError_t e = hipMalloc(&p, brick_bytes);
+            if (e != hipSuccess) return hipFail(e, "hipMalloc(brick copy)");
+            e = ctlk::launchBrick(vol->view, static_cast<float*>(p), s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                (void)hipFree(p);
+                return hipFail(e, "brick kernel");
+            }
+            vol->d_brick = static_cast<float*>(p);
+        }
+        brick = vol->d_brick;
+    }
     DeviceBuffer d_maps, d_dist, d_part;
     ctlproj_status st = upload(d_maps, maps.data(), maps.size() * sizeof(ctlk::PlaneMap), s, "plane maps");
     if (st == CTLPROJ_OK) st = upload(d_dist, distances, size_t(n_dist) * sizeof(float), s, "distances");
@@ -176,7 +198,7 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
             if ((st = alloc(d_part, need * sizeof(float), "plane partial sums")) != CTLPROJ_OK) return st;
             part_floats = need;
         }
-        hipError_t e = ctlk::launchPlaneIntegrals(vol->view, d_maps.as<ctlk::PlaneMap>() + o0, d_dist.as<float>(), job,
+        hipError_t e = ctlk::launchPlaneIntegrals(vol->view, brick, d_maps.as<ctlk::PlaneMap>() + o0, d_dist.as<float>(), job,
                                                   d_part.as<float>(), std::pow(resolution, 2.0f), d_out + o0,
                                                   n_orient, s);
         if (e != hipSuccess) return hipFail(e, "plane integral kernel");

@@ -16,6 +16,8 @@
 #include "kernels.h"
 #include "sampler.h"
 
+#include <algorithm>
+
 namespace ctlk {
 
 namespace {
@@ -104,6 +106,72 @@ __global__ __launch_bounds__(kBlock) void sample_kernel(VolumeView vol, Ranges r
                                  r.scale[2] * (p[2] - r.lo[2]) + 0.5f);
 }
 
+// Reads of the plane kernel: the resident volume's own texels, or its FP32 4x4x2 brick copy.
+template <Storage ST, Layout LY> struct TexelFetch {
+    VolumeView vol;
+    __device__ __forceinline__ float operator()(const float nhi[3], float qx, float qy, float qz) const
+    {
+        return sampleTexel<ST, LY>(makeRsrc(vol.data, vol.bytes), pitchOf(vol), nhi, qx, qy, qz);
+    }
+};
+
+// Brick copy: the padded volume in bricks of 4x4x2 FP32 voxels (one 128-byte line each), bricks
+// x fastest. A plane cuts a brick in a patch of about 4x4 voxels whatever its orientation, where
+// it cuts rows of x-fastest texels in one or two voxels each.
+struct BrickFetch {
+    const float* data;
+    uint32_t bytes;
+    uint32_t row, slab; // bytes per row of bricks (nbx * 128) and per slab (nbx * nby * 128)
+    __device__ __forceinline__ float operator()(const float nhi[3], float qx, float qy, float qz) const
+    {
+        // branch-free like sampleTexel: outside positions read voxel 0 and are discarded
+        const bool in = qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2];
+        qx = in ? qx : 0.0f;
+        qy = in ? qy : 0.0f;
+        qz = in ? qz : 0.0f;
+        const __amdgpu_buffer_rsrc_t r = makeRsrc(data, bytes);
+        const uint32_t ix = uint32_t(int(qx)), iy = uint32_t(int(qy)), iz = uint32_t(int(qz));
+        const float wx = __builtin_amdgcn_fractf(qx), wy = __builtin_amdgcn_fractf(qy);
+        const float wz = __builtin_amdgcn_fractf(qz);
+        const uint32_t x0 = ((ix >> 2) << 7) + ((ix & 3u) << 2), x1 = (((ix + 1) >> 2) << 7) + (((ix + 1) & 3u) << 2);
+        const uint32_t y0 = __umul24(iy >> 2, row) + ((iy & 3u) << 4);
+        const uint32_t y1 = __umul24((iy + 1) >> 2, row) + (((iy + 1) & 3u) << 4);
+        const uint32_t z0 = (iz >> 1) * slab + ((iz & 1u) << 6), z1 = ((iz + 1) >> 1) * slab + (((iz + 1) & 1u) << 6);
+        auto at = [&](uint32_t o) { return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0)); };
+        const float v000 = at(x0 + y0 + z0), v100 = at(x1 + y0 + z0), v010 = at(x0 + y1 + z0), v110 = at(x1 + y1 + z0);
+        const float v001 = at(x0 + y0 + z1), v101 = at(x1 + y0 + z1), v011 = at(x0 + y1 + z1), v111 = at(x1 + y1 + z1);
+        // the interleaved FP32 sampler's order: x, then y, then z
+        const float c00 = lerp(v000, v100, wx), c01 = lerp(v001, v101, wx);
+        const float c10 = lerp(v010, v110, wx), c11 = lerp(v011, v111, wx);
+        const float v = lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        return in ? v : 0.0f;
+    }
+};
+
+// brick copy of a padded volume: element i = 32 * brick + (x & 3) + 4 (y & 3) + 16 (z & 1)
+__global__ __launch_bounds__(kBlock) void brick_kernel(VolumeView src, uint32_t nbx, uint32_t nby, size_t total,
+                                                       float* __restrict__ dst)
+{
+    const uint32_t px = src.n[0] + 2, py = src.n[1] + 2, pz = src.n[2] + 2;
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < total; i += size_t(gridDim.x) * kBlock) {
+        const size_t b = i >> 5;
+        const uint32_t in = uint32_t(i & 31);
+        const uint32_t bx = uint32_t(b % nbx), by = uint32_t((b / nbx) % nby), bz = uint32_t(b / (size_t(nbx) * nby));
+        const uint32_t x = bx * 4 + (in & 3u), y = by * 4 + ((in >> 2) & 3u), z = bz * 2 + (in >> 4);
+        float v = 0.0f;
+        if (x < px && y < py && z < pz) {
+            const size_t j = size_t(z) * src.pitch_z + size_t(y) * src.pitch_y + x;
+            if (src.storage == Storage::F32)
+                v = static_cast<const float*>(src.data)[src.layout == Layout::Interleaved ? 2 * j : j];
+            else {
+                const uint16_t h = static_cast<const uint16_t*>(src.data)[src.layout == Layout::Interleaved ? 4 * j : j];
+                v = src.storage == Storage::F16 ? float(__builtin_bit_cast(_Float16, h)) : __uint_as_float(uint32_t(h) << 16);
+            }
+        }
+        dst[i] = v;
+    }
+}
+
 // planeIntegral.cl: u = H (x, y, 0, 1)^T + d * shift for pixel (x, y) and distance d, read at the
 // texel position q = u + 1/2 (the 1/2 is folded into H's last column).
 // Logical workgroup b = (distance chunk, orientation, split), orientation fastest after the split.
@@ -113,8 +181,9 @@ __global__ __launch_bounds__(kBlock) void sample_kernel(VolumeView vol, Ranges r
 // together in time. A wavefront skips a tile when the box spanned by its corner pixels and the
 // chunk's distance range misses the volume's support on some axis (with half a texel to spare):
 // every sample of such a tile reads the zero border.
-template <Storage ST, Layout LY>
-__global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const PlaneMap* __restrict__ maps,
+template <class FETCH>
+__global__ __launch_bounds__(kBlock) void plane_kernel(FETCH fetch, uint32_t n0, uint32_t n1, uint32_t n2,
+                                                       const PlaneMap* __restrict__ maps,
                                                        const float* __restrict__ dist, PlaneJob job,
                                                        float* __restrict__ partial)
 {
@@ -143,9 +212,7 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const Pla
         dhi = fmaxf(dhi, dd[j]);
         acc[j] = 0.0f;
     }
-    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
-    const Pitch p = pitchOf(vol);
-    const float nhi[3] = { float(vol.n[0]) + 1.0f, float(vol.n[1]) + 1.0f, float(vol.n[2]) + 1.0f };
+    const float nhi[3] = { float(n0) + 1.0f, float(n1) + 1.0f, float(n2) + 1.0f };
     // constant part and distance term's range of each texel coordinate
     float elo[3], ehi[3];
 #pragma unroll
@@ -174,11 +241,11 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(VolumeView vol, const Pla
         const float b0 = fmaf(h[0][0], fx, fmaf(h[0][1], fy, h[0][2]));
         const float b1 = fmaf(h[1][0], fx, fmaf(h[1][1], fy, h[1][2]));
         const float b2 = fmaf(h[2][0], fx, fmaf(h[2][1], fy, h[2][2]));
+        // every lane reads all kPlaneDists positions (a short chunk repeats its first distance and
+        // drops those sums), with no branch between the samples
 #pragma unroll
         for (int j = 0; j < kPlaneDists; ++j)
-            if (uint32_t(j) < nd)
-                acc[j] += sampleTexel<ST, LY>(rs, p, nhi, fmaf(dd[j], s[0], b0), fmaf(dd[j], s[1], b1),
-                                              fmaf(dd[j], s[2], b2));
+            acc[j] += fetch(nhi, fmaf(dd[j], s[0], b0), fmaf(dd[j], s[1], b1), fmaf(dd[j], s[2], b2));
     }
     __shared__ float red[kBlock / 64][kPlaneDists];
 #pragma unroll
@@ -276,7 +343,27 @@ hipError_t launchSamplePoints(const VolumeView& vol, const float ranges[6], cons
     return hipGetLastError();
 }
 
-hipError_t launchPlaneIntegrals(const VolumeView& vol, const PlaneMap* d_maps, const float* d_dist,
+bool brickShape(const uint32_t n[3], uint32_t nb[3], size_t* bytes)
+{
+    for (int k = 0; k < 3; ++k) nb[k] = (n[k] + 2 + (k == 2 ? 1 : 3)) / (k == 2 ? 2 : 4);
+    const uint64_t b = uint64_t(nb[0]) * nb[1] * nb[2] * 128;
+    if (b >= (1ull << 32) - 256 || uint64_t(nb[0]) * 128 >= (1ull << 24)) return false;
+    *bytes = size_t(b);
+    return true;
+}
+
+hipError_t launchBrick(const VolumeView& vol, float* d_dst, hipStream_t s)
+{
+    uint32_t nb[3];
+    size_t bytes = 0;
+    if (!brickShape(vol.n, nb, &bytes)) return hipErrorInvalidValue;
+    const size_t total = bytes / 4;
+    const size_t blocks = std::min<size_t>((total + kBlock - 1) / kBlock, 1u << 20);
+    brick_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(vol, nb[0], nb[1], total, d_dst);
+    return hipGetLastError();
+}
+
+hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, const PlaneMap* d_maps, const float* d_dist,
                                 const PlaneJob& job, float* d_partial, float scale, float* d_out,
                                 uint32_t out_stride, hipStream_t s)
 {
@@ -286,7 +373,30 @@ hipError_t launchPlaneIntegrals(const VolumeView& vol, const PlaneMap* d_maps, c
     const uint64_t blocks = chunks * job.n_orient * job.splits;
     if (blocks > 0x7fffffffull || uint64_t(job.n_orient) * job.n_dist > 0xffffffffull - kBlock)
         return hipErrorInvalidConfiguration;
-    CTLK_DISPATCH(plane_kernel, vol, uint32_t(blocks), vol, d_maps, d_dist, job, d_partial);
+    const uint32_t n0 = vol.n[0], n1 = vol.n[1], n2 = vol.n[2];
+    if (d_brick) {
+        uint32_t nb[3];
+        size_t bytes = 0;
+        if (!brickShape(vol.n, nb, &bytes)) return hipErrorInvalidValue;
+        const BrickFetch f{ d_brick, uint32_t(bytes), nb[0] * 128u, nb[0] * nb[1] * 128u };
+        plane_kernel<BrickFetch><<<uint32_t(blocks), kBlock, 0, s>>>(f, n0, n1, n2, d_maps, d_dist, job, d_partial);
+    } else {
+        const bool il = vol.layout == Layout::Interleaved;
+        switch (vol.storage) {
+        case Storage::F16:
+            if (il) plane_kernel<TexelFetch<Storage::F16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            else plane_kernel<TexelFetch<Storage::F16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            break;
+        case Storage::BF16:
+            if (il) plane_kernel<TexelFetch<Storage::BF16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            else plane_kernel<TexelFetch<Storage::BF16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            break;
+        default:
+            if (il) plane_kernel<TexelFetch<Storage::F32, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            else plane_kernel<TexelFetch<Storage::F32, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            break;
+        }
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const uint32_t rb = uint32_t((uint64_t(job.n_orient) * job.n_dist + kBlock - 1) / kBlock);

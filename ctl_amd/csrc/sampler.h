@@ -155,13 +155,15 @@ __device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& 
 // CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP) = sampleTexel at q = u + 1/2: zero wherever all eight
 // taps are border texels, that is unless 0 <= q < n + 1 on every axis (nhi = n + 1). NaN
 // coordinates read zero.
+// Branch-free, so that the loads of several samples can be in flight together: a position
+// outside the support reads texel 0 instead and its value is discarded.
 template <Storage ST, Layout LY>
 __device__ __forceinline__ float sampleTexel(__amdgpu_buffer_rsrc_t rs, const Pitch& p, const float nhi[3],
                                              float qx, float qy, float qz)
 {
-    if (!(qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2]))
-        return 0.0f;
-    return trilinear<ST, LY>(rs, p, qx, qy, qz);
+    const bool in = qx >= 0.0f && qy >= 0.0f && qz >= 0.0f && qx < nhi[0] && qy < nhi[1] && qz < nhi[2];
+    const float v = trilinear<ST, LY>(rs, p, in ? qx : 0.0f, in ? qy : 0.0f, in ? qz : 0.0f);
+    return in ? v : 0.0f;
 }
 
 template <Storage ST, Layout LY>

@@ -164,6 +164,28 @@ def test_plane_integrals_against_oracle(storage, layout):
     assert np.all(got[-1] == 0.0)
 
 
+@pytest.mark.parametrize("storage", STORAGES)
+def test_plane_integrals_brick_and_texel_reads_agree(storage, monkeypatch):
+    """The plane kernel reads an FP32 brick copy of the volume by default and the resident texels
+    with CTLPROJ_PLANE_READS=texels: the same sample positions and interpolation order, so FP32
+    agrees bit for bit; FP16 interpolates its halves in another FMA order (within 1e-6)."""
+    vol = random_volume(31, shape=(26, 22, 30))
+    rng = np.random.default_rng(32)
+    normals = random_normals(rng, 7)
+    dist = rng.uniform(-12, 12, 10).astype(np.float32)
+    bricks = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
+    monkeypatch.setenv("CTLPROJ_PLANE_READS", "texels")
+    texels = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
+    if storage == "f16":
+        assert np.abs(bricks - texels).max() <= 1e-6 * np.abs(texels).max()
+    else:
+        assert np.array_equal(bricks, texels)
+    side, reso = 48, 0.8
+    r32 = oracle.plane_integrals(rounded(vol.data, storage), vol.voxel_size, vol.offset, normals, dist, side, side,
+                                 reso)
+    assert np.abs(bricks - r32).max() <= 2e-5 * np.abs(r32).max()
+
+
 def test_plane_integrals_axis_layer_sums_and_single_integral():
     rng = np.random.default_rng(11)
     X, Y, Z = 10, 12, 6

@@ -25,11 +25,11 @@ HBM_PEAK_GBPS = 8000.0
 L1_PEAK_GBPS = 256 * 2.4 * 64.0
 
 # HBM traffic per launch of the 360-view FP32 projection (LDS-staged kernel), from the rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_f32_256cube_360views_1024x768.md, in
+# --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_kernel_f32_256cube_360views_1024x768.md, in
 # KiB per dispatch, each scaled by what the counter reads for a plain 1 GiB device copy
 # (bench.py --copy-calibration: FETCH_SIZE 524300 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
-FETCH_KIB, FETCH_CAL = 5.0576e7, 1048576 / 524300
-WRITE_KIB, WRITE_CAL = 1.1383e6, 1.0
+FETCH_KIB, FETCH_CAL = 5.0309e7, 1048576 / 524300
+WRITE_KIB, WRITE_CAL = 1.1412e6, 1.0
 TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
 # LDS read bandwidth of gfx950: 256 CUs x 2.4 GHz x 128 B/clk.
 LDS_PEAK_GBPS = 256 * 2.4 * 128.0
@@ -312,7 +312,7 @@ def bench_processing(args, torch, w, ctl, oracle):
         roofline={"bound": "l1", "achieved": gather, "peak": L1_PEAK_GBPS, "unit": "GB/s",
                   "frac": gather / L1_PEAK_GBPS,
                   "note": "achieved = samples inside the support x 8 taps x 4 B per wall second; "
-                          "the kernel is bound by cache-line traffic beyond L2 (counters in "
+                          "the kernel is bound by the vector-L1 line rate of its gathers (counters in "
                           "profiles/plane_kernel_f32_256cube_1024orient_128dist.md)"})}
     log(f"radon3d: {json.dumps(res['radon3d'])}")
     del ones

@@ -227,7 +227,7 @@ def bench_served(args, torch, W, ctl):
     srv.wait(rid)
     res = {"phantoms": "8 x 256^3 FP16 resident", "mix": "80% DRR 512x512, 20% scan 180x768x768"}
     n = 60 if args.quick else 120
-    for rate in (200, 1000):
+    for rate in (200, 500, 1000, 2000):  # the arrival rates of the served workload
         gaps = rng.exponential(1.0 / rate, n)
         kinds = rng.uniform(size=n) < 0.8
         geoms = [drr_pose() if k else None for k in kinds]

@@ -155,15 +155,17 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
     if (!(env && std::strcmp(env, "texels") == 0) && ctlk::brickShape(vol->n, nb, &brick_bytes)) {
         if (!vol->d_brick) {
             void* p = nullptr;
-            hipError_t e = hipMalloc(&p, brick_bytes);
-            if (e != hipSuccess) return hipFail(e, "hipMalloc(brick copy)");
-            e = ctlk::launchBrick(vol->view, static_cast<float*>(p), s);
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e != hipSuccess) {
-                (void)hipFree(p);
-                return hipFail(e, "brick kernel");
+            if (hipMalloc(&p, brick_bytes) == hipSuccess) {
+                hipError_t e = ctlk::launchBrick(vol->view, static_cast<float*>(p), s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) {
+                    (void)hipFree(p);
+                    return hipFail(e, "brick kernel");
+                }
+                vol->d_brick = static_cast<float*>(p);
+            } else {
+                (void)hipGetLastError(); // no room for the copy: read the resident texels
             }
-            vol->d_brick = static_cast<float*>(p);
         }
         brick = vol->d_brick;
     }

@@ -149,3 +149,32 @@ def test_dynamic_general_volume_view_by_view():
     static = ext.project(VoxelVolume(w.volume, (1.0, 1.0, 1.0))).data
     full, _ = oracle.project(w.volume, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), pm, 256, 128, 0.1)
     assert oracle.rel_l2(full, static) <= 1e-4
+
+
+def test_spectral_effects_bypass_and_errors():
+    """Without spectral information anywhere the extension is the nested projector
+    (canBypassExtension, spectraleffectsextension.cpp:169-190); a spectral detector response with
+    non-spectral volumes is an error, as in the reference; parameters keep the reference's key."""
+    w = W.small(views=4)
+    setup = small_setup()
+    nested = RayCasterProjector()
+    nested.settings().raySampling = 0.1
+    ext = SpectralEffectsExtension(nested)
+    ext.configure(setup)
+    plain = VoxelVolume(w.volume, (1.0, 1.0, 1.0))
+    got = ext.project(plain).data
+    ref = RayCasterProjector()
+    ref.settings().raySampling = 0.1
+    ref.configure(setup)
+    assert np.array_equal(got, ref.project(plain).data)
+    ext.setDetectorResponse([1.0])
+    with pytest.raises(RuntimeError):
+        ext.project(plain)
+    assert ext.parameter() == {"Sampling resolution": 10.0}
+    ext.setParameter({"Sampling resolution": 2.5})
+    assert ext.parameter() == {"Sampling resolution": 2.5}
+    # a spectrum for the wrong number of views is rejected
+    ext.setDetectorResponse(None)
+    ext.setSpectralInformation(SpectralInformation([50.0], np.ones((1, 3))))
+    with pytest.raises(ValueError):
+        ext.project(SpectralVolume(plain, [0.2]))

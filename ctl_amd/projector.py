@@ -63,8 +63,22 @@ class ProjectionData:
     def view(self, v: int) -> np.ndarray:
         return self.data[v]
 
+    def module(self, v: int, m: int) -> np.ndarray:
+        """Module m of view v as a (rows, channels) array: pixel (x, y) is [y, x]."""
+        return self.data[v, m]
+
     def toVector(self) -> np.ndarray:
         return self.data.reshape(-1)
+
+    @classmethod
+    def fromVector(cls, values, nbChannels: int, nbRows: int, nbModules: int) -> "ProjectionData":
+        """ProjectionData::setDataFromVector (projectiondata.cpp:134-170): the views follow one
+        another, each module after module, each row after row, channels fastest."""
+        a = np.ascontiguousarray(values, np.float32).reshape(-1)
+        per_view = nbChannels * nbRows * nbModules
+        if per_view == 0 or a.size % per_view:
+            raise ValueError("the vector does not hold a whole number of views")
+        return cls(a.reshape(a.size // per_view, nbModules, nbRows, nbChannels))
 
     @property
     def dimensions(self):

@@ -300,9 +300,10 @@ void launchLocked(ctlproj_server* S, Slot& slot)
     unsigned long long tiles = 0;
     uint32_t n = 0;
     slot.chunks.clear();
-    // single-view requests (DRRs) first, so they do not wait behind the chunks of long scans;
-    // then the remaining chunks in arrival order
-    for (int pass = 0; pass < 2; ++pass)
+    // single-view requests (DRRs) form a launch of their own, so that they finish in well under a
+    // millisecond instead of with the chunks of long scans; without any, the chunks go in arrival
+    // order
+    for (int pass = 0; pass < 2 && !(pass == 1 && n > 0); ++pass)
         for (auto it = S->ready.begin(); it != S->ready.end() && n < kSlotItems;) {
             Request* r = it->r;
             const uint64_t px = uint64_t(it->nv) * r->M * r->R * r->C;
@@ -455,7 +456,7 @@ ctlproj_status ctlproj_server_create(ctlproj_context* ctx, size_t hbm_budget, ui
     auto* S = new (std::nothrow) ctlproj_server;
     if (!S) return fail(CTLPROJ_ERR_OUT_OF_MEMORY, "host allocation failed");
     S->ctx = ctx;
-    S->max_launch_pixels = max_launch_pixels ? max_launch_pixels : (uint64_t(64) << 20);
+    S->max_launch_pixels = max_launch_pixels ? max_launch_pixels : (uint64_t(16) << 20);
     hbm_budget = hbm_budget / kAlign * kAlign;
     if ((e = hipMalloc(reinterpret_cast<void**>(&S->arena_base), hbm_budget)) != hipSuccess) {
         delete S;

@@ -236,7 +236,8 @@ typedef struct ctlproj_server_stats {
 } ctlproj_server_stats;
 
 /* hbm_budget: bytes of the arena (outputs + geometry of admitted requests).
- * max_launch_pixels: pixels (views x modules x rows x channels) per launch, 0 = 64 Mi. */
+ * max_launch_pixels: pixels (views x modules x rows x channels) per launch, 0 = 16 Mi (a few ms of
+ * work: single-view requests wait for at most that, and run in launches of their own). */
 ctlproj_status ctlproj_server_create(ctlproj_context* ctx, size_t hbm_budget,
                                      uint64_t max_launch_pixels, ctlproj_server** out);
 /* Waits for every submitted request, then stops the server. */

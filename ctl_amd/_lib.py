@@ -156,6 +156,8 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_diag_gather_probe": (_i, [_vp, _vp, _i, _vp, _sz, _f32p, _f64p]),
     "ctlproj_spectral_extinction": (_i, [_vp, _vp, _u32, _u32, _sz, _f32p, _f64p, _f32p, _u32, _f64p, _vp]),
     "ctlproj_combine_views": (_i, [_vp, _vp, _f32p, _vp, _f32p, _u32, _sz, _vp]),
+    "ctlproj_project_spectral": (_i, [_vp, C.POINTER(_vp), _u32, _f32p, _f64p, _f32p, _u32, _f64p, _f32p, _sz]),
+    "ctlproj_project_linear_dynamic": (_i, [_vp, _vp, _vp, _f64p, _f32p, _sz]),
     "ctlproj_slicer_default_side": (_i, [_u32p, _u32p]),
     "ctlproj_slice": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _f32p, _sz]),
     "ctlproj_slice_device": (_i, [_vp, _f64p, _f64p, _u32, _u32, _u32, C.c_float, _vp, _sz]),

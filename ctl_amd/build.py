@@ -67,7 +67,7 @@ def build(verbose: bool = False) -> Path:
           "-pthread"])
     os.replace(tmp, lib)
     # the C++ mirror's demo/test programs, linked against the library
-    for name in ("projector_demo", "processing_demo"):
+    for name in ("projector_demo", "processing_demo", "extensions_demo"):
         demo = ROOT / "tests" / "cpp" / f"{name}.cpp"
         if demo.exists():
             _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(demo), "-o",

@@ -92,3 +92,81 @@ ctlproj_status ctlproj_combine_views(ctlproj_context* ctx, const float* device_a
                                      views, pixels_per_view, device_out, s);
     return finish(e, s, "combine kernel");
 }
+
+namespace {
+
+// Projections of n volumes with the configured projector, one after another into d_proj.
+ctlproj_status projectAll(ctlproj_projector* p, const ctlproj_volume* const* vols, uint32_t n, float* d_proj,
+                          size_t per_proj)
+{
+    for (uint32_t k = 0; k < n; ++k) {
+        ctlproj_status st = ctlproj_project_device(p, vols[k], d_proj + k * per_proj, per_proj);
+        if (st != CTLPROJ_OK) return st;
+    }
+    return ctlproj_projector_synchronize(p);
+}
+
+ctlproj_status copyOut(const float* d, float* host, size_t n)
+{
+    hipError_t e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "result copy");
+}
+
+} // namespace
+
+ctlproj_status ctlproj_project_spectral(ctlproj_projector* p, const ctlproj_volume* const* densities,
+                                        uint32_t n_materials, const float* mu, const double* intensity,
+                                        const float* response, uint32_t n_bins, const double* total_intensity,
+                                        float* out, size_t out_count)
+{
+    if (!p || !p->ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "projector is NULL");
+    if (!p->configured) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "project() called before configure()");
+    if (n_materials == 0 || n_materials > ctlk::kMaxMaterials)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "1 to 8 materials are supported");
+    if (!densities) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volumes is NULL");
+    const size_t per = size_t(p->V) * p->M * p->R * p->C;
+    if (out_count != per) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output size does not match the configured geometry");
+    if (per == 0) return CTLPROJ_OK;
+    if (!out) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output is NULL");
+    ctlproj_status st = begin(p->ctx);
+    if (st != CTLPROJ_OK) return st;
+    DeviceBuffer d_proj, d_out;
+    if ((st = deviceAlloc(d_proj, n_materials * per * sizeof(float), "material projections")) != CTLPROJ_OK ||
+        (st = deviceAlloc(d_out, per * sizeof(float), "spectral output")) != CTLPROJ_OK)
+        return st;
+    if ((st = projectAll(p, densities, n_materials, d_proj.as<float>(), per)) != CTLPROJ_OK) return st;
+    const size_t pixels = per / p->V;
+    if ((st = ctlproj_spectral_extinction(p->ctx, d_proj.as<float>(), n_materials, p->V, pixels, mu, intensity,
+                                          response, n_bins, total_intensity, d_out.as<float>())) != CTLPROJ_OK)
+        return st;
+    return copyOut(d_out.as<float>(), out, per);
+}
+
+ctlproj_status ctlproj_project_linear_dynamic(ctlproj_projector* p, const ctlproj_volume* slope,
+                                              const ctlproj_volume* offset, const double* time_stamps, float* out,
+                                              size_t out_count)
+{
+    if (!p || !p->ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "projector is NULL");
+    if (!p->configured) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "project() called before configure()");
+    if (!slope || !offset) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume is NULL");
+    for (int k = 0; k < 3; ++k)
+        if (slope->n[k] != offset->n[k])
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "offset and slope volumes have different sizes");
+    const size_t per = size_t(p->V) * p->M * p->R * p->C;
+    if (out_count != per) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "output size does not match the configured geometry");
+    if (per == 0) return CTLPROJ_OK;
+    if (!out || !time_stamps) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    ctlproj_status st = begin(p->ctx);
+    if (st != CTLPROJ_OK) return st;
+    DeviceBuffer d_proj;
+    if ((st = deviceAlloc(d_proj, 2 * per * sizeof(float), "dynamic projections")) != CTLPROJ_OK) return st;
+    const ctlproj_volume* vols[2] = { slope, offset };
+    if ((st = projectAll(p, vols, 2, d_proj.as<float>(), per)) != CTLPROJ_OK) return st;
+    // time stamps enter as float, as LinearDynamicVolume::updateVolume does (float(time()))
+    std::vector<float> ca(p->V), cb(p->V, 1.0f);
+    for (uint32_t v = 0; v < p->V; ++v) ca[v] = float(time_stamps[v]);
+    float* a = d_proj.as<float>();
+    if ((st = ctlproj_combine_views(p->ctx, a, ca.data(), a + per, cb.data(), p->V, per / p->V, a)) != CTLPROJ_OK)
+        return st;
+    return copyOut(a, out, per);
+}

@@ -172,6 +172,21 @@ ctlproj_status ctlproj_combine_views(ctlproj_context* ctx, const float* device_a
                                      const float* device_b, const float* coeff_b, uint32_t views,
                                      size_t pixels_per_view, float* device_out);
 
+/* Host-level forms of the two above for callers without device buffers, on the configured
+ * projector: the material densities are projected into internal device memory, combined there and
+ * the extinction ([V][M][R][C]) copied to `out`.
+ * ctlproj_project_spectral: SpectralEffectsExtension::projectLinear with the spectrum and
+ * attenuation given as for ctlproj_spectral_extinction (n_materials densities).
+ * ctlproj_project_linear_dynamic: DynamicProjectorExtension::project of a LinearDynamicVolume
+ * (value = slope * t + offset at view v's time stamp t = time_stamps[v], taken as float). */
+ctlproj_status ctlproj_project_spectral(ctlproj_projector* p, const ctlproj_volume* const* densities,
+                                        uint32_t n_materials, const float* mu, const double* intensity,
+                                        const float* response, uint32_t n_bins, const double* total_intensity,
+                                        float* out, size_t out_count);
+ctlproj_status ctlproj_project_linear_dynamic(ctlproj_projector* p, const ctlproj_volume* slope,
+                                              const ctlproj_volume* offset, const double* time_stamps, float* out,
+                                              size_t out_count);
+
 ctlproj_status ctlproj_projector_synchronize(ctlproj_projector* p);
 
 /* Duration of the last ray-casting kernel on the projector's stream (HIP events), in ms. */

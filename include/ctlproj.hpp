@@ -206,6 +206,16 @@ public:
         return sum;
     }
 
+    // the C handle with the current settings applied, for the extensions below
+    ctlproj_projector* prepared()
+    {
+        pushSettings();
+        return h_;
+    }
+    ViewDimensions viewDimensions() const { return viewDim_; }
+    uint32_t nbViews() const { return nbViews_; }
+    Context& context() const { return *ctx_; }
+
 private:
     static void progressThunk(int view, void* self)
     {
@@ -230,6 +240,84 @@ private:
     Settings settings_;
     ViewDimensions viewDim_;
     uint32_t nbViews_ = 0;
+};
+
+// SpectralEffectsExtension (projectors/spectraleffectsextension.cpp:232-352) with the ray caster as
+// nested projector: one projection per material density and the sum over energy bins on the
+// device. The spectrum (per-bin, per-view intensities; I0 per view) and each material's mass
+// attenuation coefficient per bin (cm^2/g) are inputs; the detector response is optional.
+class SpectralEffectsExtension {
+public:
+    explicit SpectralEffectsExtension(RayCasterProjector& nested) : nested_(nested) {}
+    // intensities: [bin][view]; totalIntensity: one value per view (empty: the sum over bins)
+    void setSpectralInformation(uint32_t nbBins, std::vector<double> intensities, std::vector<double> totalIntensity = {})
+    {
+        bins_ = nbBins;
+        inten_ = std::move(intensities);
+        total_ = std::move(totalIntensity);
+    }
+    void setDetectorResponse(std::vector<float> response) { resp_ = std::move(response); }
+    bool isLinear() const { return false; }
+    // volumes[k] is a density volume, massAttenuation[k] its coefficients, one per bin
+    ProjectionData projectComposite(const std::vector<const ResidentVolume*>& volumes,
+                                    const std::vector<std::vector<float>>& massAttenuation)
+    {
+        const uint32_t nm = uint32_t(volumes.size());
+        const uint32_t V = nested_.nbViews();
+        if (massAttenuation.size() != nm) throw std::domain_error("one coefficient list per material");
+        if (inten_.size() != size_t(bins_) * V) throw std::domain_error("spectral information does not match the views");
+        std::vector<float> mu(size_t(bins_) * nm);
+        for (uint32_t k = 0; k < nm; ++k) {
+            if (massAttenuation[k].size() != bins_) throw std::domain_error("one coefficient per energy bin");
+            for (uint32_t b = 0; b < bins_; ++b) mu[size_t(b) * nm + k] = massAttenuation[k][b] * 0.1f; // 1/cm -> 1/mm
+        }
+        std::vector<double> total = total_;
+        if (total.empty()) {
+            total.assign(V, 0.0);
+            for (uint32_t b = 0; b < bins_; ++b)
+                for (uint32_t v = 0; v < V; ++v) total[v] += inten_[size_t(b) * V + v];
+        }
+        std::vector<const ctlproj_volume*> h(nm);
+        for (uint32_t k = 0; k < nm; ++k) h[k] = volumes[k]->get();
+        ProjectionData ret;
+        ret.viewDim = nested_.viewDimensions();
+        ret.nbViews = V;
+        ret.data.resize(size_t(V) * ret.viewDim.nbModules * ret.viewDim.nbRows * ret.viewDim.nbChannels);
+        check(ctlproj_project_spectral(nested_.prepared(), h.data(), nm, mu.data(), inten_.data(),
+                                       resp_.empty() ? nullptr : resp_.data(), bins_, total.data(), ret.data.data(),
+                                       ret.data.size()));
+        return ret;
+    }
+
+private:
+    RayCasterProjector& nested_;
+    uint32_t bins_ = 0;
+    std::vector<double> inten_, total_;
+    std::vector<float> resp_;
+};
+
+// DynamicProjectorExtension (projectors/dynamicprojectorextension.cpp:39-66) for a
+// LinearDynamicVolume (img/lineardynamicvolume.cpp:194-198: value = slope * t + offset): view v
+// sees the volume at timeStamps[v].
+class DynamicProjectorExtension {
+public:
+    explicit DynamicProjectorExtension(RayCasterProjector& nested) : nested_(nested) {}
+    ProjectionData project(const ResidentVolume& slope, const ResidentVolume& offset,
+                           const std::vector<double>& timeStamps)
+    {
+        const uint32_t V = nested_.nbViews();
+        if (timeStamps.size() != V) throw std::domain_error("one time stamp per view");
+        ProjectionData ret;
+        ret.viewDim = nested_.viewDimensions();
+        ret.nbViews = V;
+        ret.data.resize(size_t(V) * ret.viewDim.nbModules * ret.viewDim.nbRows * ret.viewDim.nbChannels);
+        check(ctlproj_project_linear_dynamic(nested_.prepared(), slope.get(), offset.get(), timeStamps.data(),
+                                             ret.data.data(), ret.data.size()));
+        return ret;
+    }
+
+private:
+    RayCasterProjector& nested_;
 };
 
 // Chunk2D<float> (img/chunk2d.h): width x height, x fastest.

@@ -178,3 +178,35 @@ def test_spectral_effects_bypass_and_errors():
     ext.setSpectralInformation(SpectralInformation([50.0], np.ones((1, 3))))
     with pytest.raises(ValueError):
         ext.project(SpectralVolume(plain, [0.2]))
+
+
+def test_cpp_mirror_extensions_program(tmp_path):
+    """The C++ mirrors of SpectralEffectsExtension and DynamicProjectorExtension
+    (include/ctlproj.hpp over ctlproj_project_spectral / ctlproj_project_linear_dynamic), driven
+    like CTL code, against the oracle."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent.parent / "ctl_amd" / "lib" / "extensions_demo"
+    out = tmp_path / "ext.f32"
+    res = subprocess.run([str(exe), str(out)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    raw = np.fromfile(out, np.float32)
+    nv = 64 ** 3
+    n_proj = 4 * 128 * 256
+    vol = raw[:nv].reshape(64, 64, 64)
+    spec = raw[nv:nv + n_proj].reshape(4, 1, 128, 256)
+    dyn = raw[nv + n_proj:].reshape(4, 1, 128, 256)
+    bone = np.ascontiguousarray(vol[:, :, ::-1] * np.float32(0.5))
+    slope = vol * np.float32(0.02)
+    pm = GeometryEncoder.encodeFullGeometry(small_setup())
+    one = ((1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    p1, _ = oracle.project(vol, *one, pm, 256, 128, 0.1)
+    p2, _ = oracle.project(bone, *one, pm, 256, 128, 0.1)
+    inten = np.array([[1.0, 1.1, 0.9, 1.2], [2.0, 2.0, 1.8, 2.1], [0.5, 0.4, 0.6, 0.5]])
+    mu = np.float32(0.1) * np.array([[0.4, 1.3], [0.25, 0.5], [0.18, 0.3]], np.float32)
+    ref = oracle.spectral_extinction(np.stack([p1, p2]), mu, inten, [0.8, 1.0, 0.95])
+    assert oracle.rel_l2(ref, spec) <= 1e-4
+    ts = [0.0, 10.0, 25.0, 40.0]
+    ref_d = np.concatenate([oracle.project(slope * np.float32(t) + vol, *one, pm[v:v + 1], 256, 128, 0.1)[0]
+                            for v, t in enumerate(ts)])
+    assert oracle.rel_l2(ref_d, dyn) <= 1e-4

@@ -15,7 +15,8 @@ inline size_t storageBytes(Storage s) { return s == Storage::F32 ? 4 : 2; }
 //   Plain:       one voxel per texel (4 B FP32, 2 B FP16/BF16).
 //   Interleaved: 8-byte texels that carry the neighbours a trilinear sample needs, so one 16-byte
 //                load fetches the taps of two x-adjacent texels:
-//                  FP32       texel(x,y,z) = [v(x,y,z), v(x,y,z+1)]           (2 loads/sample)
+//                  FP32       texel(x,y,z) = [v(x,y,z), v(x,y,z+1) - v(x,y,z)] (2 loads/sample;
+//                             the stored difference makes each z-lerp one FMA)
 //                  FP16/BF16  texel(x,y,z) = [v(x,y,z), v(x,y+1,z), v(x,y,z+1), v(x,y+1,z+1)]
 //                                                                              (1 load/sample)
 enum class Layout : int { Plain = 0, Interleaved = 1 };

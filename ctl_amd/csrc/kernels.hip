@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
 // of a gather through the vector L1, whose scattered cache lines bound the unstaged kernel. A
 // chunk whose box does not fit the LDS budget samples from HBM as the unstaged kernel does.
 // Samples, their order within a ray and the FP32 arithmetic of each sample are the same as in
-// castTile. The box holds the volume's own 8-byte texels: FP32 z-pairs (two 16-byte LDS reads per
+// castTile. The box holds the volume's own 8-byte texels: FP32 (v, dz) (two 16-byte LDS reads per
 // sample, rows y and y+1) or FP16/BF16 (y,z)-quads (one 16-byte read).
 
 // Wave-wide min / max: DPP reductions within each 16-lane row, then the four row results. All
@@ -364,8 +364,8 @@ __device__ __forceinline__ uint32_t wavePkMin(uint32_t v)
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Trilinear sample from the LDS box (z-pair texels) at byte offset `ofs`, with rows px8 bytes
-// apart; the same lerps in the same order as Sampler<F32, Interleaved>.
+// Trilinear sample from the LDS box ((v, dz) texels) at byte offset `ofs`, with rows px8 bytes
+// apart; the same arithmetic in the same order as Sampler<F32, Interleaved>.
 __device__ __forceinline__ float triLds(const float2* __restrict__ box, uint32_t ofs, uint32_t px8,
                                         float wx, float wy, float wz)
 {
@@ -373,9 +373,7 @@ __device__ __forceinline__ float triLds(const float2* __restrict__ box, uint32_t
     const float2* q = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(p) + px8);
     const float2 a0 = p[0], a1 = p[1];
     const float2 b0 = q[0], b1 = q[1];
-    const float c00 = lerp(a0.x, a1.x, wx), c01 = lerp(a0.y, a1.y, wx);
-    const float c10 = lerp(b0.x, b1.x, wx), c11 = lerp(b0.y, b1.y, wx);
-    return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    return triDeltaZ(a0, a1, b0, b1, wx, wy, wz);
 }
 
 // FP16/BF16 quads: texels x and x+1 at byte offset `ofs`; the same lerps as QuadHalf::tri.
@@ -385,11 +383,7 @@ __device__ __forceinline__ float quadLds(const float2* __restrict__ box, uint32_
 {
     const uint2* p = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(box) + ofs);
     const uint2 a = p[0], b = p[1];
-    const float c00 = lerpPair<ST>(a.x, false, b.x, false, wx); // y0 z0
-    const float c10 = lerpPair<ST>(a.x, true, b.x, true, wx);   // y1 z0
-    const float c01 = lerpPair<ST>(a.y, false, b.y, false, wx); // y0 z1
-    const float c11 = lerpPair<ST>(a.y, true, b.y, true, wx);   // y1 z1
-    return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+    return quadTri<ST>(a.x, a.y, b.x, b.y, wx, wy, wz);
 }
 
 // Samples [wlo, whi] of a chunk for a lane whose own range is [a, e] (`live` if non-empty), with
@@ -929,9 +923,10 @@ __device__ __forceinline__ void writeTexel(void* dst, size_t i, size_t x, size_t
         if (st == Storage::F32) static_cast<float*>(dst)[i] = v(x, y, z);
         else static_cast<uint16_t*>(dst)[i] = toBits(v(x, y, z), st);
     } else if (st == Storage::F32) {
+        // (v(z), v(z+1) - v(z)): the difference a z-lerp would form, rounded once here
         float2 t;
         t.x = v(x, y, z);
-        t.y = v(x, y, z + 1);
+        t.y = v(x, y, z + 1) - t.x;
         static_cast<float2*>(dst)[i] = t;
     } else {
         ushort4 t;

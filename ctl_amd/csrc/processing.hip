@@ -140,10 +140,9 @@ struct BrickFetch {
         auto at = [&](uint32_t o) { return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, o, 0, 0)); };
         const float v000 = at(x0 + y0 + z0), v100 = at(x1 + y0 + z0), v010 = at(x0 + y1 + z0), v110 = at(x1 + y1 + z0);
         const float v001 = at(x0 + y0 + z1), v101 = at(x1 + y0 + z1), v011 = at(x0 + y1 + z1), v111 = at(x1 + y1 + z1);
-        // the interleaved FP32 sampler's order: x, then y, then z
-        const float c00 = lerp(v000, v100, wx), c01 = lerp(v001, v101, wx);
-        const float c10 = lerp(v010, v110, wx), c11 = lerp(v011, v111, wx);
-        const float v = lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        // the interleaved FP32 sampler's arithmetic: z-differences, z, then x, then y
+        const float v = triDeltaZ(make_float2(v000, v001 - v000), make_float2(v100, v101 - v100),
+                                  make_float2(v010, v011 - v010), make_float2(v110, v111 - v110), wx, wy, wz);
         return in ? v : 0.0f;
     }
 };

@@ -51,22 +51,34 @@ __device__ __forceinline__ float lerpPair(uint32_t lo_word, bool lo_hi, uint32_t
     }
 }
 
+// Trilinear value from the four (v(z), v(z+1) - v(z)) texels around a sample: a0, a1 at x, x+1 of
+// row y, b0, b1 of row y+1. The z-lerps take the stored difference (one FMA each, the same
+// rounding as lerp), then x, then y.
+__device__ __forceinline__ float triDeltaZ(float2 a0, float2 a1, float2 b0, float2 b1, float wx, float wy,
+                                           float wz)
+{
+    const float c00 = fmaf(wz, a0.y, a0.x), c10 = fmaf(wz, a1.y, a1.x); // y0: x0, x1
+    const float c01 = fmaf(wz, b0.y, b0.x), c11 = fmaf(wz, b1.y, b1.x); // y1: x0, x1
+    return lerp(lerp(c00, c10, wx), lerp(c01, c11, wx), wy);
+}
+
 template <Storage ST, Layout LY> struct Sampler;
 
-// FP32, one voxel per texel: four 8-byte loads (x-pairs at y/z corners).
+// FP32, one voxel per texel: four 8-byte loads (x-pairs at y/z corners). The z-differences are
+// formed here, so the result is the interleaved sampler's bit for bit.
 template <> struct Sampler<Storage::F32, Layout::Plain> {
     static constexpr uint32_t kShift = 2;
-    static __device__ __forceinline__ float lerpx(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so, float w)
+    static __device__ __forceinline__ float2 load(__amdgpu_buffer_rsrc_t r, uint32_t ofs, uint32_t so)
     {
-        const float2 t = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0));
-        return lerp(t.x, t.y, w);
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, ofs, so, 0));
     }
     static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
                                                 float wx, float wy, float wz)
     {
-        const float c00 = lerpx(r, ofs, 0, wx), c10 = lerpx(r, ofs, p.y, wx);
-        const float c01 = lerpx(r, ofs, p.z, wx), c11 = lerpx(r, ofs, p.yz, wx);
-        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        const float2 t00 = load(r, ofs, 0), t10 = load(r, ofs, p.y); // z0: rows y, y+1
+        const float2 t01 = load(r, ofs, p.z), t11 = load(r, ofs, p.yz); // z1
+        return triDeltaZ(make_float2(t00.x, t01.x - t00.x), make_float2(t00.y, t01.y - t00.y),
+                         make_float2(t10.x, t11.x - t10.x), make_float2(t10.y, t11.y - t10.y), wx, wy, wz);
     }
 };
 
@@ -80,17 +92,16 @@ template <Storage ST> struct PlainHalf {
         const uint32_t t10 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.y, 0);
         const uint32_t t01 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.z, 0);
         const uint32_t t11 = __builtin_amdgcn_raw_buffer_load_b32(r, ofs, p.yz, 0);
-        const float c00 = lerpPair<ST>(t00, false, t00, true, wx);
-        const float c10 = lerpPair<ST>(t10, false, t10, true, wx);
-        const float c01 = lerpPair<ST>(t01, false, t01, true, wx);
-        const float c11 = lerpPair<ST>(t11, false, t11, true, wx);
-        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        // z, then x, then y, as the FP32 samplers
+        const float c00 = lerpPair<ST>(t00, false, t01, false, wz), c10 = lerpPair<ST>(t00, true, t01, true, wz);
+        const float c01 = lerpPair<ST>(t10, false, t11, false, wz), c11 = lerpPair<ST>(t10, true, t11, true, wz);
+        return lerp(lerp(c00, c10, wx), lerp(c01, c11, wx), wy);
     }
 };
 template <> struct Sampler<Storage::F16, Layout::Plain> : PlainHalf<Storage::F16> {};
 template <> struct Sampler<Storage::BF16, Layout::Plain> : PlainHalf<Storage::BF16> {};
 
-// FP32 z-pair texels: two 16-byte loads, rows y and y+1, each [v(x,z) v(x,z+1) v(x+1,z) v(x+1,z+1)].
+// FP32 (v, dz) texels: two 16-byte loads, rows y and y+1, each the texels at x and x+1.
 template <> struct Sampler<Storage::F32, Layout::Interleaved> {
     static constexpr uint32_t kShift = 3;
     static __device__ __forceinline__ float tri(__amdgpu_buffer_rsrc_t r, const Pitch& p, uint32_t ofs,
@@ -98,11 +109,21 @@ template <> struct Sampler<Storage::F32, Layout::Interleaved> {
     {
         const float4 a = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
         const float4 b = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, p.y, 0));
-        const float c00 = lerp(a.x, a.z, wx), c01 = lerp(a.y, a.w, wx); // y0: z0, z1
-        const float c10 = lerp(b.x, b.z, wx), c11 = lerp(b.y, b.w, wx); // y1: z0, z1
-        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        return triDeltaZ(make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y),
+                         make_float2(b.z, b.w), wx, wy, wz);
     }
 };
+
+// Trilinear value from the (y,z)-quads at x (words p0: z, p1: z+1) and x+1 (q0, q1), each word
+// [v(y) v(y+1)]: z, then x, then y, as the FP32 samplers.
+template <Storage ST>
+__device__ __forceinline__ float quadTri(uint32_t p0, uint32_t p1, uint32_t q0, uint32_t q1, float wx, float wy,
+                                         float wz)
+{
+    const float c00 = lerpPair<ST>(p0, false, p1, false, wz), c10 = lerpPair<ST>(q0, false, q1, false, wz); // y0
+    const float c01 = lerpPair<ST>(p0, true, p1, true, wz), c11 = lerpPair<ST>(q0, true, q1, true, wz);     // y1
+    return lerp(lerp(c00, c10, wx), lerp(c01, c11, wx), wy);
+}
 
 // FP16/BF16 (y,z)-quad texels: one 16-byte load = the quads at x and x+1, each
 // [v(y,z) v(y+1,z) | v(y,z+1) v(y+1,z+1)] as two 32-bit words.
@@ -113,11 +134,7 @@ template <Storage ST> struct QuadHalf {
     {
         typedef uint32_t u4 __attribute__((ext_vector_type(4)));
         const u4 t = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(r, ofs, 0, 0));
-        const float c00 = lerpPair<ST>(t.x, false, t.z, false, wx); // y0 z0
-        const float c10 = lerpPair<ST>(t.x, true, t.z, true, wx);   // y1 z0
-        const float c01 = lerpPair<ST>(t.y, false, t.w, false, wx); // y0 z1
-        const float c11 = lerpPair<ST>(t.y, true, t.w, true, wx);   // y1 z1
-        return lerp(lerp(c00, c10, wy), lerp(c01, c11, wy), wz);
+        return quadTri<ST>(t.x, t.y, t.z, t.w, wx, wy, wz);
     }
 };
 template <> struct Sampler<Storage::F16, Layout::Interleaved> : QuadHalf<Storage::F16> {};

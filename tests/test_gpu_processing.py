@@ -168,7 +168,7 @@ def test_plane_integrals_against_oracle(storage, layout):
 def test_plane_integrals_brick_and_texel_reads_agree(storage, monkeypatch):
     """The plane kernel reads an FP32 brick copy of the volume by default and the resident texels
     with CTLPROJ_PLANE_READS=texels: the same sample positions and interpolation order, so FP32
-    agrees bit for bit; FP16 interpolates its halves in another FMA order (within 1e-6)."""
+    and BF16 agree bit for bit; FP16 interpolates its halves in another FMA order (within 1e-6)."""
     vol = random_volume(31, shape=(26, 22, 30))
     rng = np.random.default_rng(32)
     normals = random_normals(rng, 7)

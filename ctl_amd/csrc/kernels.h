@@ -126,6 +126,7 @@ struct CastConst {
     uint32_t chunk;      // LDS-staged kernel: samples per ray and chunk (0: unstaged kernel)
     uint32_t lds_texels; // LDS-staged kernel: capacity of the LDS box in 8-byte texels
     uint32_t accumulate; // 1: add to the output instead of overwriting it
+    uint32_t lane_x, lane_y; // lane -> pixel in a wavefront tile: bit-gather codes (makeCastConst)
 };
 
 CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,

@@ -229,6 +229,20 @@ __device__ __forceinline__ float sampleAt(__amdgpu_buffer_rsrc_t rs, const Pitch
 // Each ray's own range [i0, i1] (the reference's loop bounds, trimmed to samples that can touch
 // the volume) is honoured with the exec mask in the head and tail of the wave's range; in between
 // every active ray samples, unmasked and unrolled.
+// Pixel (lx, ly) of `lane` in its wavefront tile. Each group of 16 lanes covers 8 x 2 pixels
+// (4 x 4 or 2 x 8 in narrower tiles) as four 2 x 2-pixel quads: the lanes that the LDS and the
+// texture path serve together then sample neighbouring positions. On the 256^3 / 1024 x 768
+// workload this is 5% (FP32, staged) to 8% (FP16) faster than row-major lanes, with the same
+// output. Tiles one pixel wide or tall stay row-major. Either way lx and ly each gather a subset
+// of the lane's six bits, in order: up to three runs of bits, each a shift and a mask (9 bits of
+// the code, from laneCode). Branch-free on purpose: a branch here made the compiler serialise
+// the gather loop's loads.
+__device__ __forceinline__ uint32_t gatherBits(uint32_t lane, uint32_t e)
+{
+    return ((lane >> (e & 7u)) & ((e >> 3) & 63u)) | ((lane >> ((e >> 9) & 7u)) & ((e >> 12) & 63u)) |
+           ((lane >> ((e >> 18) & 7u)) & ((e >> 21) & 63u));
+}
+
 template <Storage ST, Layout LY, bool INTERP, int UNROLL>
 __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                          const RayParams& rp, const CastConst& kc,
@@ -239,7 +253,7 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
     const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
-    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t lx = gatherBits(lane, kc.lane_x), ly = gatherBits(lane, kc.lane_y);
     const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
     const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
@@ -420,7 +434,7 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
     const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
-    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t lx = gatherBits(lane, kc.lane_x), ly = gatherBits(lane, kc.lane_y);
     const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
     const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
@@ -821,7 +835,7 @@ __global__ __launch_bounds__(kBlock) void probe_gather_kernel(VolumeView vol, co
     const uint32_t ty = t / kc.tiles_x, tx = t - ty * kc.tiles_x;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wy = wave / kc.bwx, wx = wave - wy * kc.bwx;
-    const uint32_t ly = lane / kc.wave_w, lx = lane - ly * kc.wave_w;
+    const uint32_t lx = gatherBits(lane, kc.lane_x), ly = gatherBits(lane, kc.lane_y);
     const uint32_t x = (tx * kc.bwx + wx) * kc.wave_w + lx;
     const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
@@ -1017,12 +1031,45 @@ uint32_t gridFor(size_t n)
 
 } // namespace
 
+// gatherBits code that moves the lane bits set in `mask` (in order) to the low bits
+static uint32_t laneCode(uint32_t mask)
+{
+    uint32_t e = 0, run = 0, p = 0;
+    for (uint32_t k = 0; k < 6;) {
+        if (!((mask >> k) & 1u)) {
+            ++k;
+            continue;
+        }
+        const uint32_t lo = k;
+        while (k < 6 && ((mask >> k) & 1u)) ++k;
+        const uint32_t n = k - lo;
+        e |= ((lo - p) | ((((1u << n) - 1u) << p) << 3)) << (9 * run);
+        p += n;
+        ++run; // at most three runs: the x bits of the quad layout are 0, 2.., 4..
+    }
+    return e;
+}
+
+// Lane bits that give a lane's x in a ww x wh wavefront tile (powers of two, ww * wh = 64): bit 0
+// (within a quad), the low bits of the quad index within a 16-lane group, the low bits of the
+// group index; all of the low bits for row-major tiles one pixel wide or tall.
+static uint32_t laneXMask(uint32_t ww, uint32_t wh)
+{
+    auto lg = [](uint32_t v) { uint32_t k = 0; while ((2u << k) <= v) ++k; return k; };
+    if (ww < 2 || wh < 2) return ww - 1u;
+    const uint32_t g = ww < 8 ? ww : 8;
+    return 1u | (((1u << lg(g / 2)) - 1u) << 2) | (((1u << lg(ww / g)) - 1u) << 4);
+}
+
 CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
                     const LaunchShape& sh)
 {
     CastConst kc{};
     kc.wave_w = sh.wave_w;
     kc.wave_h = sh.wave_h;
+    const uint32_t xm = laneXMask(sh.wave_w, sh.wave_h);
+    kc.lane_x = laneCode(xm);
+    kc.lane_y = laneCode(63u & ~xm);
     kc.bwx = sh.block_waves_x;
     kc.bwy = sh.block_waves_y;
     const uint32_t btw = sh.wave_w * sh.block_waves_x, bth = sh.wave_h * sh.block_waves_y;

@@ -103,16 +103,13 @@ ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::La
     }
     // a module no wider than one wavefront tile (the 16-channel modules of cylindrical detectors)
     // stacks the workgroup's wavefronts vertically instead of leaving a column of them idle; the
-    // staged kernel's taller ray bundle then needs a larger box and shorter chunks to keep its
-    // chunks inside the box (48 samples in 3840 texels: 148.6 ms against 164 ms unstacked on 40
-    // modules of 16 x 64 pixels, 239 ms with the default 64 / 2560)
+    // staged kernel's taller ray bundle then needs a larger box to keep its chunks inside it
+    // (6400 texels, three workgroups per CU: 32.6 ms for 500 views of 40 modules of 16 x 64
+    // pixels, against 36.2 ms with 48-sample chunks in 3840 texels and 38.2 ms with 7680 texels)
     if (!tuned && channels <= sh.wave_w && sh.block_waves_x > 1) {
         sh.block_waves_y *= sh.block_waves_x;
         sh.block_waves_x = 1;
-        if (staged) {
-            sh.chunk = 48;
-            sh.lds_texels = 3840;
-        }
+        if (staged) sh.lds_texels = 6400;
     }
     if (s.unroll) sh.unroll = s.unroll;
     if (s.blocks_per_cu) sh.blocks_per_cu = s.blocks_per_cu;

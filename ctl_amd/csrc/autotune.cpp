@@ -21,7 +21,7 @@ std::vector<ctlk::LaunchShape> candidates(bool staged)
         const uint32_t tiles[2][2] = { { 16, 4 }, { 8, 8 } };
         for (const auto& t : tiles)
             for (uint32_t chunk : { 64u, 48u })
-                for (uint32_t texels : { 2560u, 2048u, 3200u }) {
+                for (uint32_t texels : { 3072u, 2560u, 6400u }) {
                     ctlk::LaunchShape s = ctlk::defaultShape(true);
                     s.wave_w = t[0];
                     s.wave_h = t[1];

@@ -16,8 +16,8 @@ struct Choice {
 
 // Candidate shapes. Gathers from HBM: wavefront tiles 8x8, 16x4, 4x16 (pixels along channels x
 // rows), unroll 2, 4 and a cap of 0 (none) or 4 resident workgroups per CU. LDS-staged workloads
-// (`staged`): tiles 8x8, 16x4 with 48- or 64-sample chunks in LDS boxes of 2048, 2560 or 3200
-// texels (unroll 2), and the best unstaged shape.
+// (`staged`): tiles 8x8, 16x4 with 48- or 64-sample chunks in LDS boxes of 2560, 3072 or 6400
+// texels (unroll 4), and the default unstaged shape.
 std::vector<ctlk::LaunchShape> candidates(bool staged);
 
 // Times every candidate with `run` (returns milliseconds, or a negative value on failure), keeps

@@ -65,19 +65,20 @@ struct LaunchShape {
     // LDS staging (interleaved volumes, trilinear, one ray per pixel): samples per chunk
     // (0 = unstaged) and LDS box capacity in texels (a multiple of 128)
     uint32_t chunk = 64;
-    uint32_t lds_texels = 2560; // 20 KiB: seven workgroups per CU
+    uint32_t lds_texels = 3072; // 24 KiB: six workgroups per CU
 };
 
 // Default launch shape: 8x8 wavefront tiles unrolled 4x for the gathers from HBM; 16x4 tiles
-// unrolled 2x with 64-sample chunks in a 2560-texel LDS box for the staged kernel (the fastest
-// shapes measured on the 256^3 / 1024x768 workload).
+// unrolled 4x (a quarter of the samples gathered through L1) with 64-sample chunks in a
+// 3072-texel LDS box for the staged kernel (the fastest shapes measured on the 256^3 / 1024x768
+// workload).
 inline LaunchShape defaultShape(bool staged)
 {
     LaunchShape s;
     if (staged) {
         s.wave_w = 16;
         s.wave_h = 4;
-        s.unroll = 2;
+        s.unroll = 4;
     } else {
         s.chunk = 0;
     }

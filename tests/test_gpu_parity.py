@@ -81,13 +81,30 @@ def test_sample_count_matches_oracle(small, small_refs):
     assert abs(n - n32) <= 1e-5 * n32
 
 
+TILES = [(8, 8), (16, 4), (4, 16), (32, 2), (2, 32), (64, 1), (1, 64)]
+
+
 def test_unroll_and_tile_variants_agree(small, small_refs):
+    """Every wavefront tile shape (each maps its lanes to pixels its own way: 2 x 2 quads, or rows
+    for tiles one pixel wide or tall), staged and unstaged, against ref32."""
     r32 = small_refs[0]
-    for tile in [(8, 8), (16, 4), (4, 16), (32, 2), (64, 1)]:
+    for tile in TILES:
         for unroll in (1, 2, 4):
             got, _, _ = run_gpu(small, pm=small.pmats[::6], tile=tile, unroll=unroll)
-            e = oracle.rel_l2(r32[::6], got)
-            assert e <= L2_F32, (tile, unroll, e)
+            e, m = oracle.rel_l2(r32[::6], got), oracle.max_abs_rel(r32[::6], got)
+            assert e <= L2_F32 and m <= MAR_F32, (tile, unroll, e, m)
+        got, _, _ = run_gpu(small, pm=small.pmats[::6], tile=tile, unroll=4, lds_chunk=-1)
+        e, m = oracle.rel_l2(r32[::6], got), oracle.max_abs_rel(r32[::6], got)
+        assert e <= L2_F32 and m <= MAR_F32, (tile, "unstaged", e, m)
+
+
+@pytest.mark.parametrize("storage,bar", [("f16", L2_F16), ("bf16", L2_BF16)])
+def test_tile_variants_half_storage(small, small_refs, storage, bar):
+    r64 = small_refs[2][::6]
+    for tile in TILES:
+        got, _, _ = run_gpu(small, storage, pm=small.pmats[::6], tile=tile, unroll=4)
+        e = oracle.rel_l2(r64, got)
+        assert e <= bar, (storage, tile, e)
 
 
 # Staged vs unstaged kernel: the same samples with the same per-sample arithmetic, summed in a
@@ -138,8 +155,8 @@ def test_lds_staged_real_size_oblique_views():
     w = W.real(views=360)
     pick = [0, 45, 137, 225]
     r32, _ = run_ref(w, views=pick)
-    plain, _, _ = run_gpu(w, pm=w.pmats[pick], lds_chunk=-1, tile=(16, 4), unroll=2)
-    got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 2
+    plain, _, _ = run_gpu(w, pm=w.pmats[pick], lds_chunk=-1, tile=(16, 4), unroll=4)
+    got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 4
     check_same_samples(plain, got, f"lds real views {pick}")
     e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
     e0 = oracle.rel_l2(r32, plain)

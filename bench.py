@@ -312,7 +312,8 @@ def bench_processing(args, torch, w, ctl, oracle):
         roofline={"bound": "l1", "achieved": gather, "peak": L1_PEAK_GBPS, "unit": "GB/s",
                   "frac": gather / L1_PEAK_GBPS,
                   "note": "achieved = samples inside the support x 8 taps x 4 B per wall second; "
-                          "the kernel is bound by the vector-L1 line rate of its gathers (counters in "
+                          "the kernel is bound by the vector-L1 line rate of its gathers together "
+                          "with the last-level cache its brick copy is read from (counters in "
                           "profiles/plane_kernel_f32_256cube_1024orient_128dist.md)"})}
     log(f"radon3d: {json.dumps(res['radon3d'])}")
     del ones

@@ -250,15 +250,21 @@ hipError_t launchResample(const VolumeView& vol, const float ranges[6], const fl
 // d_out[i] at the point d_xyz[3i .. 3i+2].
 hipError_t launchSamplePoints(const VolumeView& vol, const float ranges[6], const float* d_xyz, size_t n,
                               float* d_out, hipStream_t s);
-// Brick copy of a padded volume for the plane kernel: FP32 bricks of 4x4x2 voxels (128 bytes),
-// nb[] bricks per axis. False if the copy would not fit one 32-bit-addressed buffer.
-bool brickShape(const uint32_t n[3], uint32_t nb[3], size_t* bytes);
-hipError_t launchBrick(const VolumeView& vol, float* d_dst, hipStream_t s);
+// FP32 brick copies of a padded volume for the plane kernel (128-byte bricks): Texel8, the
+// interleaved texels in bricks of 4x2x2; Pair16, 16-byte elements (the texels of x and x + 1) in
+// bricks of 2x2x2. Auto takes Texel8 while that copy is at most kTexel8CacheBytes (it then stays
+// in the last-level cache), Pair16 beyond.
+enum class BrickKind : int { Auto = 0, Texel8 = 1, Pair16 = 2 };
+constexpr uint64_t kTexel8CacheBytes = 192ull << 20;
+// The kind brickShape resolves `want` to, nb[] bricks per axis and the copy's size. False if the
+// copy would not fit one 32-bit-addressed buffer.
+bool brickShape(const uint32_t n[3], BrickKind want, BrickKind* kind, uint32_t nb[3], size_t* bytes);
+hipError_t launchBrick(const VolumeView& vol, BrickKind kind, float* d_dst, hipStream_t s);
 // Plane integrals: d_partial holds n_orient * n_dist * splits floats; d_out[d * out_stride + o]
-// receives scale * (the sum over the template plane). d_brick (the volume's brick copy, or NULL
-// to read the resident texels) selects the reads.
-hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, const PlaneMap* d_maps,
-                                const float* d_dist, const PlaneJob& job, float* d_partial, float scale,
-                                float* d_out, uint32_t out_stride, hipStream_t s);
+// receives scale * (the sum over the template plane). d_brick (the volume's brick copy of kind
+// brick_kind, or NULL to read the resident texels) selects the reads.
+hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, BrickKind brick_kind,
+                                const PlaneMap* d_maps, const float* d_dist, const PlaneJob& job, float* d_partial,
+                                float scale, float* d_out, uint32_t out_stride, hipStream_t s);
 
 } // namespace ctlk

@@ -147,22 +147,30 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
         return finish(e, s, "zero fill");
     }
     // the volume's brick copy (made once, kept with the volume); without it the kernel reads the
-    // resident texels. CTLPROJ_PLANE_READS=texels keeps the texel reads.
+    // resident texels. CTLPROJ_PLANE_READS=texels keeps the texel reads; =pairs asks for the
+    // Pair16 copy whatever the volume's size (a copy of the other kind is replaced).
     const float* brick = nullptr;
     const char* env = std::getenv("CTLPROJ_PLANE_READS");
+    const ctlk::BrickKind want = (env && std::strcmp(env, "pairs") == 0) ? ctlk::BrickKind::Pair16 : ctlk::BrickKind::Auto;
     uint32_t nb[3];
     size_t brick_bytes = 0;
-    if (!(env && std::strcmp(env, "texels") == 0) && ctlk::brickShape(vol->n, nb, &brick_bytes)) {
+    ctlk::BrickKind kind = ctlk::BrickKind::Auto;
+    if (!(env && std::strcmp(env, "texels") == 0) && ctlk::brickShape(vol->n, want, &kind, nb, &brick_bytes)) {
+        if (vol->d_brick && vol->brick_kind != kind) {
+            (void)hipFree(vol->d_brick);
+            vol->d_brick = nullptr;
+        }
         if (!vol->d_brick) {
             void* p = nullptr;
             if (hipMalloc(&p, brick_bytes) == hipSuccess) {
-                hipError_t e = ctlk::launchBrick(vol->view, static_cast<float*>(p), s);
+                hipError_t e = ctlk::launchBrick(vol->view, kind, static_cast<float*>(p), s);
                 if (e == hipSuccess) e = hipStreamSynchronize(s);
                 if (e != hipSuccess) {
                     (void)hipFree(p);
                     return hipFail(e, "brick kernel");
                 }
                 vol->d_brick = static_cast<float*>(p);
+                vol->brick_kind = kind;
             } else {
                 (void)hipGetLastError(); // no room for the copy: read the resident texels
             }
@@ -200,7 +208,8 @@ ctlproj_status planeIntegrals(const ctlproj_volume* vol, const std::vector<ctlk:
             if ((st = alloc(d_part, need * sizeof(float), "plane partial sums")) != CTLPROJ_OK) return st;
             part_floats = need;
         }
-        hipError_t e = ctlk::launchPlaneIntegrals(vol->view, brick, d_maps.as<ctlk::PlaneMap>() + o0, d_dist.as<float>(), job,
+        hipError_t e = ctlk::launchPlaneIntegrals(vol->view, brick, vol->brick_kind, d_maps.as<ctlk::PlaneMap>() + o0,
+                                                  d_dist.as<float>(), job,
                                                   d_part.as<float>(), std::pow(resolution, 2.0f), d_out + o0,
                                                   n_orient, s);
         if (e != hipSuccess) return hipFail(e, "plane integral kernel");

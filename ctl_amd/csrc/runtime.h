@@ -27,8 +27,9 @@ struct ctlproj_volume {
     uint32_t n[3] = { 0, 0, 0 };
     float voxel_mm[3] = { 0, 0, 0 };
     float offset_mm[3] = { 0, 0, 0 };
-    // FP32 brick copy for the plane integrals, made on first use (guarded by ctx->mu)
+    // FP32 brick copy for the plane integrals and its kind, made on first use (guarded by ctx->mu)
     mutable float* d_brick = nullptr;
+    mutable ctlk::BrickKind brick_kind = ctlk::BrickKind::Auto;
 };
 
 struct ctlproj_projector {

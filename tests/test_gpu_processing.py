@@ -166,24 +166,48 @@ def test_plane_integrals_against_oracle(storage, layout):
 
 @pytest.mark.parametrize("storage", STORAGES)
 def test_plane_integrals_brick_and_texel_reads_agree(storage, monkeypatch):
-    """The plane kernel reads an FP32 brick copy of the volume by default and the resident texels
-    with CTLPROJ_PLANE_READS=texels: the same sample positions and interpolation order, so FP32
-    and BF16 agree bit for bit; FP16 interpolates its halves in another FMA order (within 1e-6)."""
+    """The plane kernel reads an FP32 brick copy of the volume by default (8-byte texel bricks for
+    a volume this small; CTLPROJ_PLANE_READS=pairs asks for the 16-byte pair bricks that larger
+    volumes get) and the resident texels with CTLPROJ_PLANE_READS=texels: the same sample positions
+    and interpolation order, so FP32 and BF16 agree bit for bit; FP16 interpolates its halves in
+    another FMA order (within 1e-6)."""
     vol = random_volume(31, shape=(26, 22, 30))
     rng = np.random.default_rng(32)
     normals = random_normals(rng, 7)
     dist = rng.uniform(-12, 12, 10).astype(np.float32)
     bricks = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
+    monkeypatch.setenv("CTLPROJ_PLANE_READS", "pairs")
+    pairs = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
     monkeypatch.setenv("CTLPROJ_PLANE_READS", "texels")
     texels = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
-    if storage == "f16":
-        assert np.abs(bricks - texels).max() <= 1e-6 * np.abs(texels).max()
-    else:
-        assert np.array_equal(bricks, texels)
+    for got in (bricks, pairs):
+        if storage == "f16":
+            assert np.abs(got - texels).max() <= 1e-6 * np.abs(texels).max()
+        else:
+            assert np.array_equal(got, texels)
     side, reso = 48, 0.8
     r32 = oracle.plane_integrals(rounded(vol.data, storage), vol.voxel_size, vol.offset, normals, dist, side, side,
                                  reso)
     assert np.abs(bricks - r32).max() <= 2e-5 * np.abs(r32).max()
+
+
+def test_plane_integrals_brick_kind_switch_on_one_volume(monkeypatch):
+    """One resident volume asked for pair bricks after its texel bricks were made (the copy is
+    replaced) and back: every result equals the texel reads bit for bit."""
+    vol = random_volume(33, shape=(20, 24, 18))
+    rng = np.random.default_rng(34)
+    normals = random_normals(rng, 5)
+    dist = rng.uniform(-10, 10, 9).astype(np.float32)
+    rt = RadonTransform3D(vol)
+    first = rt.planeIntegrals(normals, dist)
+    monkeypatch.setenv("CTLPROJ_PLANE_READS", "pairs")
+    second = rt.planeIntegrals(normals, dist)
+    monkeypatch.delenv("CTLPROJ_PLANE_READS")
+    third = rt.planeIntegrals(normals, dist)
+    monkeypatch.setenv("CTLPROJ_PLANE_READS", "texels")
+    texels = rt.planeIntegrals(normals, dist)
+    for got in (first, second, third):
+        assert np.array_equal(got, texels)
 
 
 def test_plane_integrals_axis_layer_sums_and_single_integral():

@@ -101,16 +101,8 @@ ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::La
         sh.wave_w = s.tile_w;
         sh.wave_h = s.tile_h;
     }
-    // a module no wider than one wavefront tile (the 16-channel modules of cylindrical detectors)
-    // stacks the workgroup's wavefronts vertically instead of leaving a column of them idle; the
-    // staged kernel's taller ray bundle then needs a larger box to keep its chunks inside it
-    // (6400 texels, three workgroups per CU: 32.6 ms for 500 views of 40 modules of 16 x 64
-    // pixels, against 36.2 ms with 48-sample chunks in 3840 texels and 38.2 ms with 7680 texels)
-    if (!tuned && channels <= sh.wave_w && sh.block_waves_x > 1) {
-        sh.block_waves_y *= sh.block_waves_x;
-        sh.block_waves_x = 1;
-        if (staged) sh.lds_texels = 6400;
-    }
+    // narrow modules stack the wavefronts (the autotuner's candidates include stacked shapes)
+    if (!tuned) sh = ctlk::narrowModuleShape(sh, channels, staged);
     if (s.unroll) sh.unroll = s.unroll;
     if (s.blocks_per_cu) sh.blocks_per_cu = s.blocks_per_cu;
     if (s.lds_chunk) sh.chunk = s.lds_chunk < 0 ? 0u : uint32_t(s.lds_chunk);
@@ -830,7 +822,7 @@ ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_vo
     const bool staged = stagedWorkload(p->s, ev.view.storage, ev.view.layout);
     const bool ok = ctltune::tune(ctltune::key(ev.view.storage, ev.view.layout, p->s.interpolate ? 1 : 0,
                                                p->C, p->R, staged),
-                                  run, &best, staged);
+                                  run, &best, staged, p->C);
     (void)hipFree(d_g);
     (void)hipFree(d_o);
     if (!ok) return fail(CTLPROJ_ERR_DEVICE, "autotuning failed: no candidate ran");

@@ -12,9 +12,28 @@ namespace ctltune {
 namespace {
 std::mutex g_mu;
 std::map<std::string, Choice> g_table;
+
+// The candidate shapes before the stacked forms for narrow modules are added.
+std::vector<ctlk::LaunchShape> candidateList(bool staged);
+
 } // namespace
 
-std::vector<ctlk::LaunchShape> candidates(bool staged)
+std::vector<ctlk::LaunchShape> candidates(bool staged, uint32_t channels)
+{
+    std::vector<ctlk::LaunchShape> out = candidateList(staged);
+    if (channels == 0) return out;
+    std::vector<ctlk::LaunchShape> all;
+    for (const auto& s : out) {
+        all.push_back(s);
+        const ctlk::LaunchShape t = ctlk::narrowModuleShape(s, channels, staged && s.chunk != 0);
+        if (t.block_waves_x != s.block_waves_x) all.push_back(t);
+    }
+    return all;
+}
+
+namespace {
+
+std::vector<ctlk::LaunchShape> candidateList(bool staged)
 {
     std::vector<ctlk::LaunchShape> out;
     if (staged) {
@@ -48,12 +67,14 @@ std::vector<ctlk::LaunchShape> candidates(bool staged)
     return out;
 }
 
+} // namespace
+
 bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
-          Choice* best, bool staged)
+          Choice* best, bool staged, uint32_t channels)
 {
     Choice b;
     b.ms = -1.0f;
-    for (const auto& s : candidates(staged)) {
+    for (const auto& s : candidates(staged, channels)) {
         run(s); // first launch of a shape warms its code object and caches
         const float ms = run(s);
         if (ms < 0.0f) continue;
@@ -99,6 +120,7 @@ std::string exportTable()
         first = false;
         const auto& s = kv.second.shape;
         os << "\"" << kv.first << "\":{\"tile_w\":" << s.wave_w << ",\"tile_h\":" << s.wave_h
+           << ",\"waves_x\":" << s.block_waves_x << ",\"waves_y\":" << s.block_waves_y
            << ",\"unroll\":" << s.unroll << ",\"blocks_per_cu\":" << s.blocks_per_cu << ",\"lds_chunk\":" << s.chunk
            << ",\"lds_texels\":" << s.lds_texels << ",\"ms\":" << kv.second.ms << "}";
     }

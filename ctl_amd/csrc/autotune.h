@@ -18,12 +18,14 @@ struct Choice {
 // rows), unroll 2, 4 and a cap of 0 (none) or 4 resident workgroups per CU. LDS-staged workloads
 // (`staged`): tiles 8x8, 16x4 with 48- or 64-sample chunks in LDS boxes of 2560, 3072 or 6400
 // texels (unroll 4), and the default unstaged shape.
-std::vector<ctlk::LaunchShape> candidates(bool staged);
+// With `channels` > 0 (single projections), each candidate that narrowModuleShape changes for
+// modules of that width is followed by its stacked form.
+std::vector<ctlk::LaunchShape> candidates(bool staged, uint32_t channels = 0);
 
 // Times every candidate with `run` (returns milliseconds, or a negative value on failure), keeps
 // the fastest and records it in the process-wide table under `key`.
 bool tune(const std::string& key, const std::function<float(const ctlk::LaunchShape&)>& run,
-          Choice* best, bool staged = false);
+          Choice* best, bool staged = false, uint32_t channels = 0);
 
 // Shape recorded for `key` by an earlier tune(); false if none.
 bool lookup(const std::string& key, ctlk::LaunchShape* shape);

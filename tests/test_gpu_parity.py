@@ -204,6 +204,30 @@ def test_helical_multimodule():
     assert e <= L2_F32 and m <= MAR_F32
 
 
+def test_helical_autotuned_shape():
+    """16-channel modules: the autotuner also times the candidates with their wavefronts stacked
+    (one column of 1 x 4 wavefronts per module); whichever it keeps, the projection stays within
+    the FP32 bar and the recorded shape is a valid wavefront layout."""
+    import json
+    from ctl_amd import autotune_table
+    pm = W.helical_geometry(n_views=12, turns=0.05)
+    vol = np.ascontiguousarray(W.ellipsoid_phantom((128, 128, 64), (2.8, 2.8, 4.0), 12, 3002,
+                                                   mu=(0.0, 0.04)))
+    w = W.Workload("helical small", vol, (2.8, 2.8, 4.0), (0.0, 0.0, 0.0), pm, 16, 64, 0.25)
+    r32, _ = run_ref(w)
+    p = RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((pm, w.channels, w.rows))
+    rv = ResidentVolume(VoxelVolume(w.volume, w.voxel_mm, w.offset_mm))
+    assert p.autotune(rv) > 0.0
+    table = json.loads(autotune_table())
+    assert table and all(v["waves_x"] * v["waves_y"] == 4 for v in table.values())
+    got = p.project(rv).data
+    e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+    print(f"helical autotuned {table}: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= L2_F32 and m <= MAR_F32
+
+
 def test_real_size_views_against_oracle():
     w = W.real(views=360)
     pick = [0, 137]

@@ -204,6 +204,31 @@ def test_helical_multimodule():
     assert e <= L2_F32 and m <= MAR_F32
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_odd_sizes_anisotropic_voxels_offset_volume(storage):
+    """Odd volume sizes (odd padded pitches), anisotropic voxels (per-axis step scaling), a volume
+    offset from the isocentre and a detector whose sides are no multiple of any wavefront tile
+    (partial tiles on both edges): staged and unstaged FP32 against ref32, half storage against
+    ref64."""
+    dims, vs, off = (37, 29, 41), (1.3, 0.9, 1.7), (6.0, -4.5, 9.0)
+    vol = np.ascontiguousarray(W.ellipsoid_phantom(dims, vs, 9, 3101, semi_axes=(3.0, 14.0)))
+    pm = W.carm_circle(9, (77, 53), (1.1, 0.9))
+    w = W.Workload("odd", vol, vs, off, pm, 77, 53, 0.15)
+    if storage == "f32":
+        r32, _ = run_ref(w)
+        for kw in ({}, {"lds_chunk": -1}, {"tile": (16, 4), "lds_chunk": 16, "lds_texels": 1024}):
+            got, _, _ = run_gpu(w, **kw)
+            e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
+            print(f"odd sizes f32 {kw}: rel L2 {e:.3e}, max abs rel {m:.3e}")
+            assert e <= L2_F32 and m <= MAR_F32, (kw, e, m)
+    else:
+        r64, _ = run_ref(w, fp64=True)
+        got, _, _ = run_gpu(w, storage)
+        e = oracle.rel_l2(r64, got)
+        print(f"odd sizes {storage}: rel L2 vs ref64 {e:.3e}")
+        assert e <= (L2_F16 if storage == "f16" else L2_BF16)
+
+
 def test_helical_autotuned_shape():
     """16-channel modules: the autotuner also times the candidates with their wavefronts stacked
     (one column of 1 x 4 wavefronts per module); whichever it keeps, the projection stays within

@@ -197,9 +197,10 @@ ctlproj_status ctlproj_projector_last_kernel_ms(ctlproj_projector* p, float* ms)
 ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlproj_volume* vol,
                                                uint64_t* n_samples);
 
-/* Measures the candidate launch shapes (wavefront pixel tile 8x8 / 16x4 / 4x16, ray-step unroll
- * 2 / 4) on up to 24 of the configured views and remembers the fastest for this workload class
- * (storage, layout, mode, detector size); later projections with tile/unroll 0 use it.
+/* Measures the candidate launch shapes (wavefront pixel tile, ray-step unroll, LDS chunk and box;
+ * for narrow modules also with the wavefronts stacked) on up to 24 of the configured views and
+ * remembers the fastest for this workload class (storage, layout, mode, detector size); later
+ * projections with automatic tuning fields use it.
  * No counterpart in the reference (it has no launch-shape choice). */
 ctlproj_status ctlproj_projector_autotune(ctlproj_projector* p, const ctlproj_volume* vol,
                                          float* best_ms);

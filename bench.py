@@ -400,6 +400,8 @@ def main() -> int:
                     help="real workload with the gathers from HBM through L1 (no LDS staging)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="with --only-real: one timed projection per variant and nothing else (profiling)")
+    ap.add_argument("--only-helical", action="store_true",
+                    help="only the helical workload (2000 views of 40 modules; profiling)")
     ap.add_argument("--only-processing", action="store_true",
                     help="only the volume processing leg (3D Radon, slicer, resampler; profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -417,7 +419,7 @@ def main() -> int:
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported here. The "
-              "projector shards over views with no collective (see DESIGN.md section 7).",
+              "projector would shard over views with no collective; this build runs on one GPU.",
               file=sys.stderr)
         return 2
 
@@ -432,6 +434,9 @@ def main() -> int:
     if args.only_processing:
         res = bench_processing(args, torch, W.real(views=1), ctl, oracle)
         print(json.dumps({"processing": res}), flush=True)
+        return 0
+    if args.only_helical:
+        print(json.dumps({"helical": bench_helical(args, torch, W, ctl, oracle)}), flush=True)
         return 0
     w, real = bench_real(args, torch, W, ctl, oracle)
     extra = {}

@@ -4,8 +4,9 @@
 // geometry) fits into the server's arena; admission uploads the P-matrices and runs the FP64
 // per-view setup, then splits the request into chunks of views. A scheduler thread forms
 // persistent-kernel launches from the ready chunks of all admitted requests (any volume, any
-// detector size, one storage type and mode per launch) on two alternating streams, so a launch
-// can start while the previous one drains. A completion thread retires launches, copies finished
+// detector size, one storage type and mode per launch) on three launch slots, each with its own
+// stream: scan launches queue one behind the other on the device and leave part of it free, and
+// single-view requests (DRRs) take any free slot, so that they run beside a scan launch. A completion thread retires launches, copies finished
 // requests to the caller's host arrays, releases their arena blocks and wakes waiters.
 #include "ctlproj.h"
 #include "kernels.h"
@@ -31,7 +32,13 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr size_t kAlign = 256;
-constexpr uint32_t kSlots = 2;
+// Launches in flight: at most kSlots - 1 of them carry scan chunks, so that a single-view request
+// (DRR) always finds a slot.
+constexpr uint32_t kSlots = 3;
+// A launch with scan chunks uses the device's workgroup capacity less 1/kScanFree of it, so that the
+// workgroups of a DRR launch behind it start at once; scan launches run one after another on the
+// device (each waits on the event of the one before), so that the free part stays free.
+constexpr uint32_t kScanFree = 8;
 constexpr uint32_t kSlotItems = 4096;
 
 size_t alignUp(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
@@ -122,6 +129,7 @@ struct Slot {
     hipEvent_t done = nullptr;
     std::vector<Chunk> chunks;
     bool busy = false;
+    bool drr = false; // single-view requests only
 };
 
 using GroupKey = std::tuple<int, int, int, int>; // storage, layout, interpolate, LDS-staged
@@ -287,10 +295,15 @@ void failChunksLocked(ctlproj_server* S, std::vector<Chunk>& chunks, ctlproj_sta
     retireLocked(S, chunks);
 }
 
-// Forms one launch from the ready chunks of the first ready chunk's group.
-void launchLocked(ctlproj_server* S, Slot& slot)
+// Forms one launch from the ready chunks of the first ready chunk's group (with drr_only, of the
+// first single-view request's group, taking single-view requests only).
+void launchLocked(ctlproj_server* S, Slot& slot, bool drr_only)
 {
-    const GroupKey key = groupOf(S->ready.front().r);
+    auto first = S->ready.begin();
+    if (drr_only)
+        while (first != S->ready.end() && first->r->V != 1) ++first;
+    if (first == S->ready.end()) return;
+    const GroupKey key = groupOf(first->r);
     // groups of trilinear requests with one ray per pixel run the LDS-staged kernel, the others
     // the unstaged kernel
     const bool staged = std::get<3>(key) != 0;
@@ -303,7 +316,7 @@ void launchLocked(ctlproj_server* S, Slot& slot)
     // single-view requests (DRRs) form a launch of their own, so that they finish in well under a
     // millisecond instead of with the chunks of long scans; without any, the chunks go in arrival
     // order
-    for (int pass = 0; pass < 2 && !(pass == 1 && n > 0); ++pass)
+    for (int pass = 0; pass < (drr_only ? 1 : 2) && !(pass == 1 && n > 0); ++pass)
         for (auto it = S->ready.begin(); it != S->ready.end() && n < kSlotItems;) {
             Request* r = it->r;
             const uint64_t px = uint64_t(it->nv) * r->M * r->R * r->C;
@@ -346,17 +359,25 @@ void launchLocked(ctlproj_server* S, Slot& slot)
             e = hipStreamWaitEvent(slot.stream, c.r->prep_done, 0);
             last = c.r;
         }
+    bool drr = true;
+    for (const auto& c : slot.chunks) drr = drr && c.r->V == 1;
+    const uint32_t launch_grid = drr ? grid : std::max<uint32_t>(1, grid - grid / kScanFree);
+    if (!drr)
+        for (const Slot& other : S->slots)
+            if (e == hipSuccess && &other != &slot && other.busy && !other.drr)
+                e = hipStreamWaitEvent(slot.stream, other.done, 0);
     if (e == hipSuccess) e = ctlk::launchQueueInit(slot.d_q, tiles, slot.stream);
     if (e == hipSuccess)
         e = ctlk::launchBatch(slot.d_items, n, ctlk::Storage(std::get<0>(key)), ctlk::Layout(std::get<1>(key)),
-                              std::get<2>(key), shape.unroll, shape.blocks_per_cu, lds_texels, grid, slot.d_q, 1,
-                              slot.stream);
+                              std::get<2>(key), shape.unroll, shape.blocks_per_cu, lds_texels, launch_grid, slot.d_q,
+                              1, slot.stream);
     if (e == hipSuccess) e = hipEventRecord(slot.done, slot.stream);
     if (e != hipSuccess) {
         failChunksLocked(S, slot.chunks, hipFail(e, "serving launch"));
         return;
     }
     slot.busy = true;
+    slot.drr = drr;
     ++S->launches;
 }
 
@@ -369,17 +390,23 @@ void schedulerMain(ctlproj_server* S)
         admitLocked(S);
         bool launched = false;
         if (!S->ready.empty()) {
-            for (uint32_t k = 0; k < kSlots; ++k) {
-                Slot& slot = S->slots[(S->next_slot + k) % kSlots];
-                if (slot.busy) continue;
-                launchLocked(S, slot);
-                S->next_slot = (S->next_slot + k + 1) % kSlots;
-                launched = true;
-                S->cv_done.notify_all(); // wake the completion thread
-                break;
-            }
+            bool drr_ready = false;
+            for (const Chunk& c : S->ready) drr_ready = drr_ready || c.r->V == 1;
+            uint32_t scans = 0;
+            for (const Slot& sl : S->slots) scans += (sl.busy && !sl.drr) ? 1u : 0u;
+            // DRRs take any free slot; scan launches at most kSlots - 1
+            if (drr_ready || scans + 1 < kSlots)
+                for (uint32_t k = 0; k < kSlots; ++k) {
+                    Slot& slot = S->slots[(S->next_slot + k) % kSlots];
+                    if (slot.busy) continue;
+                    launchLocked(S, slot, drr_ready);
+                    S->next_slot = (S->next_slot + k + 1) % kSlots;
+                    launched = slot.busy;
+                    break;
+                }
         }
-        if (!launched) S->cv_sched.wait_for(lk, std::chrono::microseconds(500));
+        if (launched) S->cv_done.notify_all(); // wake the completion thread
+        else S->cv_sched.wait_for(lk, std::chrono::microseconds(500));
     }
 }
 
@@ -465,8 +492,19 @@ ctlproj_status ctlproj_server_create(ctlproj_context* ctx, size_t hbm_budget, ui
     S->arena.init(hbm_budget);
     if ((e = hipStreamCreateWithFlags(&S->prep, hipStreamNonBlocking)) == hipSuccess)
         e = hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking);
+    // The slot streams carry a CU mask of the whole device: HIP gives such a stream a hardware
+    // queue of its own, whereas plain streams share a pool of GPU_MAX_HW_QUEUES queues (4 by
+    // default), and a DRR launch on a stream that shares a scan launch's queue waits behind it.
+    // (Such streams also order themselves against the null stream, which the server never uses.)
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->ctx->device);
+    const std::vector<uint32_t> all_cus((uint32_t(cus) + 31) / 32, ~0u);
     for (Slot& sl : S->slots) {
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess && hipExtStreamCreateWithCUMask(&sl.stream, uint32_t(all_cus.size()), all_cus.data()) != hipSuccess) {
+            (void)hipGetLastError();
+            sl.stream = nullptr;
+            e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        }
         if (e == hipSuccess) e = hipMalloc(&sl.d_items, kSlotItems * sizeof(ctlk::BatchItem));
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_items, kSlotItems * sizeof(ctlk::BatchItem), hipHostMallocDefault);
         if (e == hipSuccess) e = hipMalloc(&sl.d_q, sizeof(ctlk::BatchQueue));

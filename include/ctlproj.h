@@ -234,8 +234,9 @@ ctlproj_status ctlproj_batch_count_samples(ctlproj_batch* b, uint64_t* n_samples
 /* ---- serving (no counterpart in the reference) ---------------------------------------------
  * A request loop over resident volumes: requests are admitted when their device footprint
  * (output + geometry) fits into the server's HBM arena, split into chunks of views, and
- * batched across volumes and detector sizes into persistent-kernel launches on two alternating
- * streams, so requests that arrive while others run join the next launch and launches overlap.
+ * batched across volumes and detector sizes into persistent-kernel launches on three streams of
+ * the server's own, so requests that arrive while others run join the next launch; single-view
+ * requests run beside a scan launch. The streams order themselves against the null stream.
  * Each result is copied to the caller's host array (pinned memory makes the copy asynchronous);
  * latency is measured from submit to the result being in host memory. Volumes must outlive the
  * requests that use them. Calls may come from any thread. */

@@ -108,3 +108,27 @@ def test_wait_timeout_and_drain(phantoms):
     srv.drain()
     assert srv.wait(rid) > 0
     srv.close()
+
+
+def test_single_view_requests_pass_a_running_scan(phantoms):
+    # a long scan cut into many launches, then DRRs: the DRRs run on the free launch slot beside
+    # the scan's launches and finish first, with the same result as alone; the scan is unaffected
+    srv = Server(hbm_budget=256 << 20, max_launch_pixels=1 << 18)
+    vol = phantoms[0][1]
+    scan_pm = scan_geometry(360, 256)
+    drr_pms = [drr_geometry(300 + k) for k in range(4)]
+    scan_alone = run_alone(srv, vol, scan_pm, 256)
+    drr_alone = [run_alone(srv, vol, pm, 128) for pm in drr_pms]
+    scan_out = np.zeros_like(scan_alone)
+    drr_outs = [np.zeros_like(a) for a in drr_alone]
+    st = Settings(raySampling=0.25)
+    sid = srv.submit(vol, scan_pm, 256, 256, scan_out, st)
+    dids = [srv.submit(vol, pm, 128, 128, o, st) for pm, o in zip(drr_pms, drr_outs)]
+    drr_lat = [srv.wait(i) for i in dids]
+    scan_lat = srv.wait(sid)
+    print(f"scan {scan_lat:.2f} ms, DRRs {', '.join(f'{t:.2f}' for t in drr_lat)} ms")
+    assert max(drr_lat) < scan_lat
+    assert np.array_equal(scan_out, scan_alone)
+    for a, o in zip(drr_alone, drr_outs):
+        assert np.array_equal(a, o)
+    srv.close()

@@ -71,8 +71,7 @@ ctlproj_status validateSettings(const ctlproj_settings& s)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume upsampling factor must be <= 16");
     if ((s.tile_w || s.tile_h) && s.tile_w * s.tile_h != 64)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "tile_w * tile_h must be 64 (one wavefront)");
-    if (s.unroll != 0 && s.unroll != 1 && s.unroll != 2 && s.unroll != 4)
-        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unroll must be 0, 1, 2 or 4");
+    if (s.unroll > 4) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unroll must be 0 to 4");
     if (s.blocks_per_cu > 8)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "blocks_per_cu must be 0..8");
     if (s.lds_chunk != 0 && s.lds_chunk != -1 && (s.lds_chunk < 16 || s.lds_chunk > 1024))

@@ -40,14 +40,16 @@ std::vector<ctlk::LaunchShape> candidateList(bool staged)
         const uint32_t tiles[2][2] = { { 16, 4 }, { 8, 8 } };
         for (const auto& t : tiles)
             for (uint32_t chunk : { 64u, 48u })
-                for (uint32_t texels : { 3072u, 2560u, 6400u }) {
-                    ctlk::LaunchShape s = ctlk::defaultShape(true);
-                    s.wave_w = t[0];
-                    s.wave_h = t[1];
-                    s.chunk = chunk;
-                    s.lds_texels = texels;
-                    out.push_back(s);
-                }
+                for (uint32_t texels : { 3072u, 2560u, 6400u })
+                    for (uint32_t u : { 3u, 4u }) {
+                        ctlk::LaunchShape s = ctlk::defaultShape(true);
+                        s.wave_w = t[0];
+                        s.wave_h = t[1];
+                        s.chunk = chunk;
+                        s.lds_texels = texels;
+                        s.unroll = u;
+                        out.push_back(s);
+                    }
         out.push_back(ctlk::defaultShape(false));
         return out;
     }

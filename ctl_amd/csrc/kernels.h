@@ -69,16 +69,16 @@ struct LaunchShape {
 };
 
 // Default launch shape: 8x8 wavefront tiles unrolled 4x for the gathers from HBM; 16x4 tiles
-// unrolled 4x (a quarter of the samples gathered through L1) with 64-sample chunks in a
-// 3072-texel LDS box for the staged kernel (the fastest shapes measured on the 256^3 / 1024x768
-// workload).
+// unrolled 3x (a third of the samples gathered through L1) with 64-sample chunks in a 3072-texel
+// LDS box for the staged kernel (the fastest shapes measured on the 256^3 / 1024x768 workload:
+// 76.3 ms at 360 views against 78.6 with unroll 4).
 inline LaunchShape defaultShape(bool staged)
 {
     LaunchShape s;
     if (staged) {
         s.wave_w = 16;
         s.wave_h = 4;
-        s.unroll = 4;
+        s.unroll = 3;
     } else {
         s.chunk = 0;
     }
@@ -90,13 +90,18 @@ inline LaunchShape defaultShape(bool staged)
 // and for the staged kernel, whose ray bundle is then taller, a box of 6400 texels (three
 // workgroups per CU; 32.6 ms for 500 views of 40 modules of 16 x 64 pixels, against 36.2 ms with
 // 48-sample chunks in 3840 texels and 38.2 ms with 7680 texels). Unchanged unless the shape has
-// more than one wavefront across and channels <= its tile width.
+// more than one wavefront across and channels <= its tile width. The staged kernel keeps unroll
+// 4 there (a quarter of the samples through L1: 132.0 ms for the 2000-view helical scan against
+// 141.6 ms with unroll 3).
 inline LaunchShape narrowModuleShape(LaunchShape s, uint32_t channels, bool staged)
 {
     if (channels > s.wave_w || s.block_waves_x < 2) return s;
     s.block_waves_y *= s.block_waves_x;
     s.block_waves_x = 1;
-    if (staged) s.lds_texels = 6400;
+    if (staged) {
+        s.lds_texels = 6400;
+        s.unroll = 4;
+    }
     return s;
 }
 

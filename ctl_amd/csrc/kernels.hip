@@ -402,8 +402,8 @@ __device__ __forceinline__ float quadLds(const float2* __restrict__ box, uint32_
 
 // Samples [wlo, whi] of a chunk for a lane whose own range is [a, e] (`live` if non-empty), with
 // [wa, we] the range every live lane of the wave shares: head and tail under the exec mask, the
-// shared body unmasked and unrolled. With UNROLL = 4 the body takes its fourth sample from `g`
-// instead of `f` (the same value, read another way); `f` everywhere else.
+// shared body unmasked and unrolled, UNROLL samples a step. With UNROLL = 3 or 4 the step's last
+// sample comes from `g` instead of `f` (the same value, read another way); `f` everywhere else.
 template <int UNROLL, typename F, typename G>
 __device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, int a, int e, bool live,
                                             float* part, F&& f, G&& g)
@@ -416,7 +416,8 @@ __device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, in
         float fi = float(wa);
         for (; i + UNROLL - 1 <= we; i += UNROLL, fi += float(UNROLL)) {
 #pragma unroll
-            for (int u = 0; u < UNROLL; ++u) part[u] += (UNROLL == 4 && u == 3) ? g(fi + float(u)) : f(fi + float(u));
+            for (int u = 0; u < UNROLL; ++u)
+                part[u] += (UNROLL >= 3 && u == UNROLL - 1) ? g(fi + float(u)) : f(fi + float(u));
         }
         for (; i <= we; ++i, fi += 1.0f) part[0] += f(fi);
     }
@@ -568,9 +569,10 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     if constexpr (ST == Storage::F32) return triLds(box, ofs, px8, wx, wy, wz);
                     else return quadLds<ST>(box, ofs, wx, wy, wz);
                 }, [&](float fi) {
-                    // a quarter of the shared body gathers through the vector L1 instead (the box
-                    // was just read from there): the LDS pipe and the texture path then share the
-                    // taps, 9% faster than LDS alone on the 256^3 / 1024 x 768 workload
+                    // with unroll 3 (4) a third (a quarter) of the shared body gathers through the
+                    // vector L1 instead (the box was just read from there): the LDS pipe and the
+                    // texture path then share the taps. 256^3 / 1024 x 768, 90 views: a third
+                    // 19.0 ms, a quarter 19.5, half 21.1, none 22.7
                     return sampleAt<ST, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
                 });
             } else {
@@ -1114,6 +1116,7 @@ hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams&
 {
     switch (unroll) {
     case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    case 3: // the unstaged kernels run 3 as 4
     case 4: raycast_kernel<ST, LY, INTERP, 4><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     default: raycast_kernel<ST, LY, INTERP, 2><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     }
@@ -1126,6 +1129,7 @@ hipError_t launchStaged(const VolumeView& vol, const RayGeom* g, const RayParams
 {
     switch (unroll) {
     case 1: staged_kernel<ST, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+    case 3: staged_kernel<ST, 3><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     case 4: staged_kernel<ST, 4><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     default: staged_kernel<ST, 2><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     }
@@ -1204,7 +1208,7 @@ hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, f
 bool validShape(const LaunchShape& s)
 {
     return s.wave_w * s.wave_h == 64 && s.block_waves_x * s.block_waves_y == kBlock / 64 &&
-           (s.unroll == 1 || s.unroll == 2 || s.unroll == 4) && s.blocks_per_cu <= 8;
+           (s.unroll >= 1 && s.unroll <= 4) && s.blocks_per_cu <= 8;
 }
 
 hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, uint32_t channels,
@@ -1312,6 +1316,7 @@ const void* batchKernelU(uint32_t unroll)
 {
     switch (unroll) {
     case 1: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 1>);
+    case 3: // the unstaged kernels run 3 as 4
     case 4: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 4>);
     default: return reinterpret_cast<const void*>(&batch_kernel<ST, LY, INTERP, 2>);
     }
@@ -1331,6 +1336,7 @@ const void* batchStagedKernelU(uint32_t unroll)
 {
     switch (unroll) {
     case 1: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 1>);
+    case 3: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 3>);
     case 4: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 4>);
     default: return reinterpret_cast<const void*>(&batch_staged_kernel<ST, 2>);
     }

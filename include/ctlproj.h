@@ -57,7 +57,9 @@ typedef struct ctlproj_settings {
     int32_t interpolate;
     uint32_t tile_w;   /* pixels per wavefront tile along channels (0 = auto) */
     uint32_t tile_h;   /* pixels per wavefront tile along rows (0 = auto) */
-    uint32_t unroll;   /* ray-step unroll of the sampling loop (0 = auto) */
+    uint32_t unroll;   /* ray-step unroll of the sampling loop, 1-4 (0 = auto); with 3 or 4 the
+                          LDS-staged kernel takes each step's last sample through L1; the unstaged
+                          kernels run 3 as 4 */
     uint32_t blocks_per_cu; /* cap on resident workgroups per CU, 1..8 (0 = auto) */
     int32_t lds_chunk;      /* LDS staging of trilinear projections with one ray per pixel:
                                samples per chunk, 16..1024 (0 = auto: on for FP32 storage, off

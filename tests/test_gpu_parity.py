@@ -89,7 +89,7 @@ def test_unroll_and_tile_variants_agree(small, small_refs):
     for tiles one pixel wide or tall), staged and unstaged, against ref32."""
     r32 = small_refs[0]
     for tile in TILES:
-        for unroll in (1, 2, 4):
+        for unroll in (1, 2, 3, 4):
             got, _, _ = run_gpu(small, pm=small.pmats[::6], tile=tile, unroll=unroll)
             e, m = oracle.rel_l2(r32[::6], got), oracle.max_abs_rel(r32[::6], got)
             assert e <= L2_F32 and m <= MAR_F32, (tile, unroll, e, m)
@@ -156,7 +156,7 @@ def test_lds_staged_real_size_oblique_views():
     pick = [0, 45, 137, 225]
     r32, _ = run_ref(w, views=pick)
     plain, _, _ = run_gpu(w, pm=w.pmats[pick], lds_chunk=-1, tile=(16, 4), unroll=4)
-    got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 4
+    got, _, _ = run_gpu(w, pm=w.pmats[pick])  # default: staged, 16x4 tiles, unroll 3
     check_same_samples(plain, got, f"lds real views {pick}")
     e, m = oracle.rel_l2(r32, got), oracle.max_abs_rel(r32, got)
     e0 = oracle.rel_l2(r32, plain)

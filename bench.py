@@ -28,8 +28,8 @@ L1_PEAK_GBPS = 256 * 2.4 * 64.0
 # --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_kernel_f32_256cube_360views_1024x768.md, in
 # KiB per dispatch, each scaled by what the counter reads for a plain 1 GiB device copy
 # (bench.py --copy-calibration: FETCH_SIZE 524300 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
-FETCH_KIB, FETCH_CAL = 5.1373e7, 1048576 / 524300
-WRITE_KIB, WRITE_CAL = 1.1739e6, 1.0
+FETCH_KIB, FETCH_CAL = 5.1107e7, 1048576 / 524300
+WRITE_KIB, WRITE_CAL = 1.1767e6, 1.0
 TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
 # LDS read bandwidth of gfx950: 256 CUs x 2.4 GHz x 128 B/clk.
 LDS_PEAK_GBPS = 256 * 2.4 * 128.0
@@ -470,9 +470,9 @@ def main() -> int:
                              "peaks: LDS 128 B/clk/CU (lds_peak), vector L1 64 B/clk/CU "
                              "(l1_gather_peak, the ceiling of gathering without LDS). The staged "
                              "kernel is bound on chip by three pipes together: 33 VALU "
-                             "instructions per sample (1.33 per CU per clock), the LDS pipe (0.75 "
-                             "busy with bank conflicts) and the texture data path (0.89 busy: a "
-                             "quarter of the samples gather through L1)",
+                             "instructions per sample (1.36 per CU per clock), the LDS pipe (0.70 "
+                             "busy with bank conflicts) and the texture data path (0.93 busy: a "
+                             "third of the samples gather through L1)",
                      "lds_peak": LDS_PEAK_GBPS, "lds_frac": achieved / LDS_PEAK_GBPS,
                      "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS,
                      "probe_gather_GBps": f32.get("probe_gather_GBps"),

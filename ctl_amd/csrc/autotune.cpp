@@ -22,11 +22,19 @@ std::vector<ctlk::LaunchShape> candidates(bool staged, uint32_t channels)
 {
     std::vector<ctlk::LaunchShape> out = candidateList(staged);
     if (channels == 0) return out;
+    auto same = [](const ctlk::LaunchShape& x, const ctlk::LaunchShape& y) {
+        return x.wave_w == y.wave_w && x.wave_h == y.wave_h && x.block_waves_x == y.block_waves_x &&
+               x.block_waves_y == y.block_waves_y && x.unroll == y.unroll && x.blocks_per_cu == y.blocks_per_cu &&
+               x.chunk == y.chunk && x.lds_texels == y.lds_texels;
+    };
     std::vector<ctlk::LaunchShape> all;
     for (const auto& s : out) {
         all.push_back(s);
+        // stacked staged shapes take unroll 4, so two candidates can stack to the same shape
         const ctlk::LaunchShape t = ctlk::narrowModuleShape(s, channels, staged && s.chunk != 0);
-        if (t.block_waves_x != s.block_waves_x) all.push_back(t);
+        bool dup = t.block_waves_x == s.block_waves_x;
+        for (const auto& u : all) dup = dup || same(u, t);
+        if (!dup) all.push_back(t);
     }
     return all;
 }

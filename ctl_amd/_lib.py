@@ -126,6 +126,7 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_project": (_i, [_vp, _vp, _f32p, _sz, PROGRESS_FN, _vp]),
     "ctlproj_project_host_volume": (_i, [_vp, _f32p, _u32p, _f32p, _f32p, _f32p, _sz]),
     "ctlproj_project_device": (_i, [_vp, _vp, _vp, _sz]),
+    "ctlproj_project_device_noalloc": (_i, [_vp, _vp, _vp, _sz]),
     "ctlproj_projector_synchronize": (_i, [_vp]),
     "ctlproj_projector_last_kernel_ms": (_i, [_vp, _f32p]),
     "ctlproj_projector_count_samples": (_i, [_vp, _vp, _u64p]),

@@ -311,11 +311,11 @@ ctlproj_status checkProject(ctlproj_projector* p, const ctlproj_volume* vol, con
 }
 
 ctlproj_status launchCast(ctlproj_projector* p, const EffectiveVolume& ev, uint32_t v0, uint32_t nv,
-                          float* d_out, int accumulate = 0)
+                          float* d_out, int accumulate = 0, const ctlk::RayGeom* geom = nullptr)
 {
     const ctlk::RayParams rp = rayParams(p, nv);
     const ctlk::LaunchShape sh = shapeFor(p->s, ev.view.storage, ev.view.layout, p->C, p->R);
-    hipError_t e = ctlk::launchRaycast(ev.view, ev.geom, p->d_geom + size_t(v0) * p->M, rp, sh,
+    hipError_t e = ctlk::launchRaycast(ev.view, ev.geom, (geom ? geom : p->d_geom) + size_t(v0) * p->M, rp, sh,
                                        d_out, p->stream, accumulate);
     return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "raycast kernel");
 }
@@ -397,6 +397,12 @@ ctlproj_status ctlproj_context_create(int device, ctlproj_context** out)
     if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
         delete ctx;
         return hipFail(e, "context stream");
+    }
+    // the device's static scratch (allocation-free projection): its address, once per device
+    if ((e = ctlk::initScratch(device)) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return hipFail(e, "hipGetSymbolAddress(scratch)");
     }
     *out = ctx;
     return CTLPROJ_OK;
@@ -598,6 +604,41 @@ ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume
     if (ev.tmp) { // the temporary volume must outlive the kernel
         if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return hipFail(e, "hipStreamSynchronize");
     }
+    return CTLPROJ_OK;
+}
+
+static_assert(ctlk::kScratchRayGeoms == CTLPROJ_NOALLOC_MAX_VIEW_MODULES,
+              "the header's limit is the scratch's capacity");
+
+ctlproj_status ctlproj_project_device_noalloc(ctlproj_projector* p, const ctlproj_volume* vol,
+                                              float* device_out, size_t out_count)
+{
+    ctlproj_status st = checkProject(p, vol, device_out, out_count);
+    if (st != CTLPROJ_OK) return st;
+    if (p->V == 0) return CTLPROJ_OK;
+    // upsampling factors other than 1 build a temporary volume (an allocation)
+    const uint32_t f = p->s.interpolate ? p->s.volume_upsampling : 1u;
+    if (f != 1)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT,
+                    "the allocation-free projection needs volume upsampling factor 1 (got " + std::to_string(f) + ")");
+    const size_t nm = size_t(p->V) * p->M;
+    if (nm > ctlk::kScratchRayGeoms)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT,
+                    "the allocation-free projection takes at most " + std::to_string(ctlk::kScratchRayGeoms) +
+                        " (view, module) pairs, got " + std::to_string(nm));
+    if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+    auto* geom = static_cast<ctlk::RayGeom*>(ctlk::scratchPtr(p->ctx->device));
+    if (!geom) return fail(CTLPROJ_ERR_DEVICE, "the device scratch has no address (context not initialised)");
+    EffectiveVolume ev;
+    if ((st = effectiveVolume(vol, p->s, p->stream, ev)) != CTLPROJ_OK) return st;
+    // every record the launch reads is written by the setup kernel first, on the same stream
+    hipError_t e = ctlk::launchSetup(p->d_pmats, p->V, p->M, p->C, p->R, ev.geom, p->s.interpolate ? 1 : 0, geom,
+                                     p->stream);
+    if (e != hipSuccess) return hipFail(e, "setup kernel");
+    if ((e = hipEventRecord(p->ev_start, p->stream)) != hipSuccess) return hipFail(e, "hipEventRecord");
+    if ((st = launchCast(p, ev, 0, p->V, device_out, 0, geom)) != CTLPROJ_OK) return st;
+    if ((e = hipEventRecord(p->ev_stop, p->stream)) != hipSuccess) return hipFail(e, "hipEventRecord");
+    p->timed = true;
     return CTLPROJ_OK;
 }
 

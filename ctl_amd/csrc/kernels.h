@@ -234,6 +234,15 @@ hipError_t launchCombineViews(const float* d_a, const float* d_ca, const float* 
 
 bool validShape(const LaunchShape& s);
 
+// Static device scratch of the allocation-free projection (one copy per device, held until the
+// process ends): room for kScratchRayGeoms ray geometry records. initScratch(device), with that
+// device current, looks up the copy's address once; scratchPtr(device) returns it afterwards
+// (NULL before). Every user of a device's scratch shares it: calls that overlap on one device race.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+constexpr size_t kScratchRayGeoms = kScratchBytes / sizeof(RayGeom);
+hipError_t initScratch(int device);
+void* scratchPtr(int device);
+
 // ---- volume processing (processing.hip) -----------------------------------------------------
 
 // Affine map of a sampling plane, FP32 as the host casts it: sampler coordinate

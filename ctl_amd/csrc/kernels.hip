@@ -17,6 +17,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdint>
+#include <mutex>
 
 namespace ctlk {
 
@@ -1402,6 +1403,44 @@ hipError_t launchBatch(const BatchItem* d_items, uint32_t n_items, Storage st, L
     void* args[] = { const_cast<BatchItem**>(&d_items), &n_items, &d_q, &stride };
     return hipLaunchKernel(fn, dim3(grid), dim3(kBlock), args,
                            staged ? batchLds(lds_texels, blocks_per_cu) : ldsForBlocks(blocks_per_cu), s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Static device scratch of the allocation-free projection: the ray geometry records of one
+// projection. Each device holds its own copy for the life of the process. Host code never uses
+// the array's name as an address (in host code it is a stand-in, not device memory): initScratch
+// asks the runtime for the current device's address once, and every launch takes that pointer.
+
+namespace {
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+} // namespace
+
+hipError_t initScratch(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice; // the symbol's address is the current device's
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* scratchPtr(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 } // namespace ctlk

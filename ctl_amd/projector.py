@@ -539,6 +539,13 @@ class RayCasterProjector:
         self._push_settings()
         check(lib().ctlproj_project_device(self.h, volume.h, C.c_void_p(out_ptr), count))
 
+    def project_device_noalloc(self, volume: ResidentVolume, out_ptr: int, count: int) -> None:
+        """project_device without any device allocation (static per-device scratch; at most
+        CTLPROJ_NOALLOC_MAX_VIEW_MODULES (view, module) pairs, upsampling factor 1). Calls that
+        overlap on one device race."""
+        self._push_settings()
+        check(lib().ctlproj_project_device_noalloc(self.h, volume.h, C.c_void_p(out_ptr), count))
+
     def synchronize(self) -> None:
         check(lib().ctlproj_projector_synchronize(self.h))
 

@@ -136,6 +136,20 @@ ctlproj_status ctlproj_project_host_volume(ctlproj_projector* p, const float* ho
  * Call ctlproj_projector_synchronize before reading `device_out`. */
 ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume* vol,
                                       float* device_out, size_t out_count);
+/* Allocation-free form of ctlproj_project_device. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during a projection (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree: the
+ * per-(view, module) ray geometry goes into a static device scratch of 8 MiB per device, held
+ * until the process ends, which limits a projection to CTLPROJ_NOALLOC_MAX_VIEW_MODULES
+ * (view, module) pairs (larger ones return CTLPROJ_ERR_INVALID_ARGUMENT, as does a volume
+ * upsampling factor other than 1, which would need a temporary volume). Same kernels and results
+ * as ctlproj_project_device; asynchronous on the projector's stream. Every call on one device
+ * shares that device's scratch: calls that overlap on one device (two projectors on two streams
+ * or threads) race and must be serialised by the caller. */
+#define CTLPROJ_NOALLOC_MAX_VIEW_MODULES 131072u
+ctlproj_status ctlproj_project_device_noalloc(ctlproj_projector* p, const ctlproj_volume* vol,
+                                              float* device_out, size_t out_count);
+
 /* AbstractProjector::projectComposite (abstractprojector.h:99-112): the sum of the projections of
  * n resident volumes (any voxel sizes and offsets), accumulated on the device in volume order on
  * the projector's stream. The host form copies the sum to `out` ([V][M][R][C]). */

@@ -197,28 +197,13 @@ def bench_served(args, torch, W, ctl):
     """8 resident 256^3 FP16 phantoms; Poisson arrivals; 80% one-view 512x512 DRRs near a
     reference C-arm pose (+-5 deg, +-10 mm), 20% 180-view 768x768 scans."""
     rng = np.random.default_rng(5)
-    phantoms = [W.ellipsoid_phantom((256, 256, 256), (0.5, 0.5, 0.5), 20, 5000 + k) for k in range(8)]
+    phantoms = W.served_phantoms()
     vols = [ctl.ResidentVolume(ctl.VoxelVolume(d, (0.5, 0.5, 0.5)), "f16") for d in phantoms]
 
     def drr_pose():
-        R = np.eye(3)
-        for ax, a in enumerate(np.deg2rad(rng.uniform(-5, 5, 3))):
-            c, s = np.cos(a), np.sin(a)
-            M = np.eye(3)
-            i, j = [(1, 2), (0, 2), (0, 1)][ax]
-            M[i, i] = c
-            M[j, j] = c
-            M[i, j] = -s
-            M[j, i] = s
-            R = R @ M
-        rot = R @ W._FIXED
-        pos = rot @ np.array([0, 0, -750.0]) + rng.uniform(-10, 10, 3)
-        setup = ctl.AcquisitionSetup(ctl.CTSystem(ctl.CarmGantry(1200.0),
-                                                  ctl.FlatPanelDetector((512, 512), (0.8, 0.8))))
-        setup.setCarmPoses(pos[None], rot.reshape(1, 9))
-        return ctl.GeometryEncoder.encodeFullGeometry(setup)
+        return W.served_drr_geometry(rng)
 
-    scan_pm = W.carm_circle(180, (768, 768), (0.533, 0.533))
+    scan_pm = W.served_scan_geometry()
     drr_out = [torch.empty(512 * 512, dtype=torch.float32).pin_memory() for _ in range(64)]
     scan_out = [torch.empty(180 * 768 * 768, dtype=torch.float32).pin_memory() for _ in range(4)]
     st = ctl.Settings(raySampling=0.2)

@@ -184,6 +184,7 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_batch_last_ms": (_i, [_vp, _f32p]),
     "ctlproj_batch_info": (_i, [_vp, _u32p, _u32p, _u32p, _u32p, _u64p]),
     "ctlproj_batch_count_samples": (_i, [_vp, _u64p]),
+    "ctlproj_batch_shape": (_i, [_vp, _vp]),
     "ctlproj_server_create": (_i, [_vp, _sz, C.c_uint64, _pp]),
     "ctlproj_server_destroy": (_i, [_vp]),
     "ctlproj_server_submit": (_i, [_vp, _vp, _f64p, _u32, _u32, _u32, _u32, C.POINTER(Settings), _vp, _sz,

@@ -335,6 +335,20 @@ ctlproj_status ctlproj_batch_info(const ctlproj_batch* b, uint32_t* tile_w, uint
     return CTLPROJ_OK;
 }
 
+ctlproj_status ctlproj_batch_shape(const ctlproj_batch* b, ctlproj_settings* settings)
+{
+    if (!b || !settings) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (!b->finalized) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "batch is not finalized");
+    settings->tile_w = b->shape.wave_w;
+    settings->tile_h = b->shape.wave_h;
+    settings->unroll = b->shape.unroll;
+    settings->blocks_per_cu = b->shape.blocks_per_cu;
+    const uint32_t box = ldsTexels(b, b->shape);
+    settings->lds_chunk = box ? int32_t(b->shape.chunk) : -1;
+    settings->lds_texels = box;
+    return CTLPROJ_OK;
+}
+
 ctlproj_status ctlproj_batch_count_samples(ctlproj_batch* b, uint64_t* n_samples)
 {
     if (!b || !n_samples) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");

@@ -43,19 +43,19 @@ struct VolumeGeom {
     float increment_mm; // min(voxel size) * raySampling
 };
 
-// Per (view, module) ray geometry written by the FP64 setup kernel. The ray direction of pixel
-// (x, y) is d = a*(x - xr) + b*(y - yr) + o (any positive scale); `c` is the source in the
-// padded texel frame: (source - corner)/voxel + 1/2.
+// Per (view, module) ray geometry written by the FP64 setup kernel: the reference's packed QR of
+// M (decomposeM, raycasterprojector.cpp:527-541: Q row-major, then R00 R01 R02 R11 R12 R22), the
+// FP32 source and volume corner in voxel units (raycasterprojector_interp.cl:35-36), and `c`, the
+// reference's corner-to-source vector plus the gathers' frame shift (sample i is at
+// fma(i, step, c) in the padded texel frame).
 struct alignas(16) RayGeom {
-    float a[3];
-    float b[3];
-    float o[3];
-    float xr, yr;
+    double qr[15];
+    float src[3];
+    float corner[3];
     float c[3];
-    uint32_t valid; // 0 if M is singular
-    float pad;
+    uint32_t valid; // 0 if any of these values is not finite (singular M)
 };
-static_assert(sizeof(RayGeom) == 64, "RayGeom is one 64-byte record");
+static_assert(sizeof(RayGeom) == 160, "RayGeom is one 160-byte record");
 
 struct LaunchShape {
     uint32_t wave_w = 8, wave_h = 8; // pixel tile per wavefront (wave_w * wave_h = 64)
@@ -238,8 +238,8 @@ bool validShape(const LaunchShape& s);
 // process ends): room for kScratchRayGeoms ray geometry records. initScratch(device), with that
 // device current, looks up the copy's address once; scratchPtr(device) returns it afterwards
 // (NULL before). Every user of a device's scratch shares it: calls that overlap on one device race.
-constexpr size_t kScratchBytes = size_t(8) << 20;
-constexpr size_t kScratchRayGeoms = kScratchBytes / sizeof(RayGeom);
+constexpr size_t kScratchRayGeoms = 131072;
+constexpr size_t kScratchBytes = kScratchRayGeoms * sizeof(RayGeom); // 20 MiB
 hipError_t initScratch(int device);
 void* scratchPtr(int device);
 

@@ -37,24 +37,65 @@ __device__ __forceinline__ uint32_t xcdRemap(uint32_t b, uint32_t nb)
 
 // ---------------------------------------------------------------------------------------------
 // FP64 setup
+//
+// One thread per (view, module) restates the reference's host-side preparation in the same
+// operations and order, so that every ray starts from the same FP64 and FP32 values the
+// reference computes: decomposeM (raycasterprojector.cpp:527-541, Householder QR of M as
+// mat::QRdecomposition / qrdcmp, matrix_algorithm.cpp:14-37, 125-171, with R negated when
+// R00 R11 R22 < 0), the source position by Cramer's rule (projectionmatrix.cpp:264-274, module 0
+// of the view, raycasterprojector.cpp:323-325) cast to FP32, and the kernel prologue's FP32
+// source, corner and corner-to-source vector (raycasterprojector_interp.cl:35-37). No
+// multiply-add is fused here (fp contract off); FP64 division and square root are IEEE.
 
 __device__ double det3(const double* m)
 {
+#pragma clang fp contract(off)
     return m[0] * (m[4] * m[8] - m[7] * m[5]) - m[1] * (m[3] * m[8] - m[6] * m[5]) +
            m[2] * (m[3] * m[7] - m[6] * m[4]);
 }
 
-__global__ void setup_kernel(const double* __restrict__ pm, uint32_t views, uint32_t modules,
-                             uint32_t channels, uint32_t rows, VolumeGeom vg, double shift,
-                             RayGeom* __restrict__ out)
+// qrdcmp of the 3x3 matrix a (row-major), in place; d receives R's diagonal.
+__device__ void householder3(double* a, double* d)
 {
+#pragma clang fp contract(off)
+    double c[2];
+    for (int k = 0; k < 2; ++k) {
+        double scale = 0.0;
+        for (int i = k; i < 3; ++i)
+            if (scale < fabs(a[3 * i + k])) scale = fabs(a[3 * i + k]);
+        if (fabs(scale) <= 1e-12) { // qFuzzyIsNull: a singular column
+            c[k] = 0.0;
+            d[k] = 0.0;
+        } else {
+            for (int i = k; i < 3; ++i) a[3 * i + k] /= scale;
+            double sum = 0.0;
+            for (int i = k; i < 3; ++i) sum += a[3 * i + k] * a[3 * i + k];
+            const double sigma = copysign(sqrt(sum), a[3 * k + k]);
+            a[3 * k + k] += sigma;
+            c[k] = sigma * a[3 * k + k];
+            d[k] = -scale * sigma;
+            for (int j = k + 1; j < 3; ++j) {
+                double t = 0.0;
+                for (int i = k; i < 3; ++i) t += a[3 * i + k] * a[3 * i + j];
+                const double tau = t / c[k];
+                for (int i = k; i < 3; ++i) a[3 * i + j] -= tau * a[3 * i + k];
+            }
+        }
+    }
+    d[2] = a[8];
+}
+
+__global__ void setup_kernel(const double* __restrict__ pm, uint32_t views, uint32_t modules,
+                             VolumeGeom vg, float shift, RayGeom* __restrict__ out)
+{
+#pragma clang fp contract(off)
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= views * modules) return;
     const uint32_t v = t / modules;
     const double* P = pm + size_t(t) * 12;
-    const double* P0 = pm + size_t(v) * modules * 12; // source from module 0 (raycasterprojector.cpp:323-325)
+    const double* P0 = pm + size_t(v) * modules * 12;
 
-    // source = null space of P0 by Cramer's rule (projectionmatrix.cpp:264-274)
+    // source = null space of P0 by Cramer's rule
     double m[9];
     const int cols[4][3] = { { 0, 1, 2 }, { 1, 2, 3 }, { 0, 2, 3 }, { 0, 1, 3 } };
     double dets[4];
@@ -66,46 +107,51 @@ __global__ void setup_kernel(const double* __restrict__ pm, uint32_t views, uint
     const double h = -dets[0];
     const double src[3] = { dets[1] / h, -dets[2] / h, dets[3] / h };
 
-    // M^-1 of this module by the adjugate, times sign(det M): the direction sign rule of
-    // decomposeM (raycasterprojector.cpp:527-541, R negated when det R < 0, det Q = +1).
+    // decomposeM: Q = Q1 Q2 from the Householder vectors, R upper with diagonal d
+    double a[9], d[3];
     for (int r = 0; r < 3; ++r)
-        for (int j = 0; j < 3; ++j) m[3 * r + j] = P[4 * r + j];
-    const double det = det3(m);
-    double adj[9];
-    adj[0] = m[4] * m[8] - m[5] * m[7];
-    adj[1] = m[2] * m[7] - m[1] * m[8];
-    adj[2] = m[1] * m[5] - m[2] * m[4];
-    adj[3] = m[5] * m[6] - m[3] * m[8];
-    adj[4] = m[0] * m[8] - m[2] * m[6];
-    adj[5] = m[2] * m[3] - m[0] * m[5];
-    adj[6] = m[3] * m[7] - m[4] * m[6];
-    adj[7] = m[1] * m[6] - m[0] * m[7];
-    adj[8] = m[0] * m[4] - m[1] * m[3];
-    const double inv_abs_det = 1.0 / fabs(det);
-
-    const double xr = 0.5 * double(channels - 1), yr = 0.5 * double(rows - 1);
-    double a[3], b[3], o[3];
-    for (int r = 0; r < 3; ++r) {
-        a[r] = adj[3 * r + 0] * inv_abs_det;
-        b[r] = adj[3 * r + 1] * inv_abs_det;
-        o[r] = a[r] * xr + b[r] * yr + adj[3 * r + 2] * inv_abs_det;
+        for (int j = 0; j < 3; ++j) a[3 * r + j] = P[4 * r + j];
+    householder3(a, d);
+    double R[6] = { d[0], a[1], a[2], d[1], a[5], d[2] }; // R00 R01 R02 R11 R12 R22
+    double u1[3] = { a[0], a[3], a[6] };
+    double u2[3] = { 0.0, a[4], a[7] };
+    const double n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    const double n2 = sqrt(u2[0] * u2[0] + u2[1] * u2[1] + u2[2] * u2[2]);
+    for (int i = 0; i < 3; ++i) {
+        u1[i] /= n1;
+        u2[i] /= n2;
     }
-    const double on = sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-    const bool ok = det != 0.0 && isfinite(det) && on > 0.0 && isfinite(h) && h != 0.0;
-
+    double Q1[9], Q2[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            Q1[3 * r + c] = (r == c ? 1.0 : 0.0) - 2.0 * u1[r] * u1[c];
+            Q2[3 * r + c] = (r == c ? 1.0 : 0.0) - 2.0 * u2[r] * u2[c];
+        }
     RayGeom g;
-    for (int r = 0; r < 3; ++r) {
-        g.a[r] = float(a[r] / on);
-        g.b[r] = float(b[r] / on);
-        g.o[r] = float(o[r] / on);
-        // volumeCorner (raycasterprojector.cpp:552-558), then the padded texel frame
-        const double corner = double(vg.offset_mm[r]) - 0.5 * double(vg.n[r]) * double(vg.voxel_mm[r]);
-        g.c[r] = float((src[r] - corner) / double(vg.voxel_mm[r]) + shift);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double q = 0.0;
+            for (int k = 0; k < 3; ++k) q += Q1[3 * r + k] * Q2[3 * k + c];
+            g.qr[3 * r + c] = q;
+        }
+    if (signbit(R[0] * R[3] * R[5]))
+        for (int i = 0; i < 6; ++i) R[i] = -R[i];
+    for (int i = 0; i < 6; ++i) g.qr[9 + i] = R[i];
+
+    // kernel prologue (FP32): source and corner in voxel units and their difference; volumeCorner
+    // (raycasterprojector.cpp:552-558) as the host computes it in FP32
+    bool ok = true;
+    for (int k = 0; k < 3; ++k) {
+        const float corner_mm = vg.offset_mm[k] - 0.5f * float(vg.n[k]) * vg.voxel_mm[k];
+        g.src[k] = float(src[k]) / vg.voxel_mm[k];
+        g.corner[k] = corner_mm / vg.voxel_mm[k];
+        // sample positions of the gathers: the reference's position plus `shift` (the sampler's
+        // -1/2 and the padding's +1)
+        g.c[k] = (g.src[k] - g.corner[k]) + shift;
+        ok = ok && isfinite(g.c[k]);
     }
-    g.xr = float(xr);
-    g.yr = float(yr);
+    for (int i = 0; i < 15; ++i) ok = ok && isfinite(g.qr[i]);
     g.valid = ok ? 1u : 0u;
-    g.pad = 0.0f;
     out[t] = g;
 }
 
@@ -128,37 +174,47 @@ __device__ __forceinline__ void checkFace(float h0, float h1, float c10, float c
     hi = (in & (lambda > hi)) ? lambda : hi;
 }
 
-// Direction, step and the reference's sample range for the sub-ray through detector
-// coordinate (px, py). The slab test restates calculateIntersections
-// (raycasterprojector_interp.cl:88-134) in the texel frame whose box is [blo, n + 1].
-// Multiply-adds of the ray setup are not fused (fp contract off, as in the oracle): otherwise the
-// compiler may fuse them differently in different kernels, and a grazing ray would gain or lose a
-// sample at its ends depending on which kernel casts it.
+// The sub-ray through detector coordinate (px, py), as the reference's kernel computes it:
+// calculateDirection in FP64 from the packed QR (raycasterprojector_interp.cl:74-86), cast to
+// FP32 and normalised (v / sqrt(dot(v, v))), scaled by the increment and divided by the voxel size
+// (:53-57); then calculateIntersections (:88-134) in the reference's own frame (source and volume
+// corner in voxel units, the box [corner - margin, corner + n + margin], 0.01-voxel relaxation)
+// and the loop bounds (uint)lambda_min .. (uint)lambda_max + 1. The operations and their order are
+// the reference's, none fused (fp contract off), so the step, the bounds and (with the gathers'
+// position frame c = source - corner + shift, computed once per view) the sample positions are
+// the reference's bit for bit. nhi = n + 1 per axis; blo = 0 selects the trilinear box (margin
+// 1/2), blo = 1 the nearest-mode box (no margin, raycasterprojector_no_interp.cl:95-140).
 __device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, float inc,
                                         const float vs[3], const float nhi[3], float blo)
 {
 #pragma clang fp contract(off)
     Ray r;
-    const float ex = px - g.xr, ey = py - g.yr;
-    float d[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) d[k] = fmaf(g.a[k], ex, fmaf(g.b[k], ey, g.o[k]));
+    const double x = double(px), y = double(py);
+    const double* q = g.qr;
+    const double qx = q[0] * x + q[3] * y + q[6];
+    const double qy = q[1] * x + q[4] * y + q[7];
+    const double qz = q[2] * x + q[5] * y + q[8];
+    const double dz = qz / q[14];
+    const double dy = (qy - dz * q[13]) / q[12];
+    const double dx = (qx - dy * q[10] - dz * q[11]) / q[9];
+    float d[3] = { float(dx), float(dy), float(dz) };
     const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) r.s[k] = (inc * (d[k] / len)) / vs[k];
 
+    const float margin = blo == 0.0f ? 0.5f : 0.0f;
     float c1[3], c2[3], l1[3], l2[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        c1[k] = blo;
-        c2[k] = nhi[k];
-        l1[k] = (c1[k] - g.c[k]) / r.s[k];
-        l2[k] = (c2[k] - g.c[k]) / r.s[k];
+        c1[k] = g.corner[k] - margin;
+        c2[k] = g.corner[k] + (nhi[k] - 1.0f) + margin;
+        l1[k] = (c1[k] - g.src[k]) / r.s[k];
+        l2[k] = (c2[k] - g.src[k]) / r.s[k];
         c1[k] -= 0.01f;
         c2[k] += 0.01f;
     }
     float lo = FLT_MAX, hi = 0.0f;
-    const float* c = g.c;
+    const float* c = g.src;
     const float* s = r.s;
     checkFace(c[1] + l1[0] * s[1], c[2] + l1[0] * s[2], c1[1], c1[2], c2[1], c2[2], l1[0], lo, hi);
     checkFace(c[0] + l1[1] * s[0], c[2] + l1[1] * s[2], c1[0], c1[2], c2[0], c2[2], l1[1], lo, hi);
@@ -174,6 +230,15 @@ __device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, fl
     r.i0 = int(fminf(lo, 1073741824.0f));
     r.i1 = int(fminf(hi, 1073741824.0f)) + 1;
     return r;
+}
+
+// Detector coordinate of sub-ray k of pixel x with ip = 1 / rays per pixel
+// (raycasterprojector_interp.cl:39-40, 52), unfused as the reference computes it.
+__device__ __forceinline__ float subRayCoord(uint32_t x, float ip, uint32_t k)
+{
+#pragma clang fp contract(off)
+    const float corner = float(x) - 0.5f + 0.5f * ip;
+    return corner + float(k) * ip;
 }
 
 // Samples whose position lies outside [blo, n+1) on any axis read only border texels and add
@@ -264,14 +329,12 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
 
     // kernel prologue of raycasterprojector_interp.cl:39-40
     const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
-    const float pcx = float(x) - 0.5f + 0.5f * ipx;
-    const float pcy = float(y) - 0.5f + 0.5f * ipy;
 
     float acc = 0.0f;
     if (g.valid) {
         for (uint32_t ra = 0; ra < rp.rpp_x; ++ra)
             for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
-                Ray ray = setupRay(g, pcx + float(ra) * ipx, pcy + float(rb) * ipy, kc.inc, kc.vs,
+                Ray ray = setupRay(g, subRayCoord(x, ipx, ra), subRayCoord(y, ipy, rb), kc.inc, kc.vs,
                                    kc.nhi, kc.blo);
                 if (!pixel) ray.hit = false;
                 tighten(ray, g.c, kc.blo, kc.nhi);
@@ -723,12 +786,10 @@ __global__ __launch_bounds__(kBlock) void count_kernel(const RayGeom* __restrict
         const uint32_t y = rem / rp.channels, x = rem - y * rp.channels;
         const RayGeom g = geom[vm];
         const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
-        const float pcx = float(x) - 0.5f + 0.5f * ipx;
-        const float pcy = float(y) - 0.5f + 0.5f * ipy;
         if (g.valid)
             for (uint32_t ra = 0; ra < rp.rpp_x; ++ra)
                 for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
-                    const Ray ray = setupRay(g, pcx + float(ra) * ipx, pcy + float(rb) * ipy, kc.inc,
+                    const Ray ray = setupRay(g, subRayCoord(x, ipx, ra), subRayCoord(y, ipy, rb), kc.inc,
                                              kc.vs, kc.nhi, kc.blo);
                     if (ray.hit && ray.i1 >= ray.i0) mine += (unsigned long long)(ray.i1 - ray.i0 + 1);
                 }
@@ -1216,12 +1277,14 @@ hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, 
                        uint32_t rows, const VolumeGeom& vg, int interpolate, RayGeom* d_geom,
                        hipStream_t s)
 {
-    // texel frame of the gathers: trilinear q = p + 1/2 (centre shift -1/2, padding +1),
-    // nearest q = p + 1 (padding +1), p = position relative to the volume corner in voxels
-    const double shift = interpolate ? 0.5 : 1.0;
+    (void)channels;
+    (void)rows;
+    // position frame of the gathers: trilinear q = p + 1/2 (the sampler's -1/2, padding +1),
+    // nearest q = p + 1 (padding +1), p = the reference's position relative to the volume corner
+    const float shift = interpolate ? 0.5f : 1.0f;
     const uint32_t n = views * modules;
-    setup_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_pmats, views, modules, channels, rows, vg, shift,
-                                              d_geom);
+    if (n == 0) return hipSuccess;
+    setup_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_pmats, views, modules, vg, shift, d_geom);
     return hipGetLastError();
 }
 

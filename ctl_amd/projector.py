@@ -716,6 +716,14 @@ class Batch:
         check(lib().ctlproj_batch_count_samples(self.h, C.byref(n)))
         return n.value
 
+    def shape_settings(self, raySampling: float) -> Settings:
+        """Settings whose tuning fields are the finalized batch's launch shape: a single
+        projection with them runs the batch's per-pixel code (same bits)."""
+        c = _lib.Settings()
+        check(lib().ctlproj_batch_shape(self.h, C.byref(c)))
+        return Settings(raySampling=raySampling, tile=(c.tile_w, c.tile_h), unroll=c.unroll,
+                        blocks_per_cu=c.blocks_per_cu, lds_chunk=c.lds_chunk, lds_texels=c.lds_texels)
+
     def close(self) -> None:
         if self.h:
             lib().ctlproj_batch_destroy(self.h)

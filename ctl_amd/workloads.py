@@ -150,3 +150,36 @@ def batched(n: int = 64, seed: int = 4000, edge=(64, 256), views=(1, 180), panel
         setup.setCarmPoses(pos, rot.reshape(V, 9))
         out.append(BatchInstance(vol, (vs, vs, vs), GeometryEncoder.encodeFullGeometry(setup), P, P))
     return out, ray_sampling
+
+
+def served_drr_geometry(rng: np.random.Generator, n: int = 512, pitch_mm: float = 0.8,
+                        sid: float = 750.0, span: float = 1200.0) -> np.ndarray:
+    """One single-view DRR of the served workload: an n x n flat panel on a C-arm (span 1200 mm)
+    whose pose is the reference pose (source on -z at `sid` from the isocentre, the C-arm
+    protocols' source frame) turned by up to +-5 deg about each axis and moved by up to +-10 mm,
+    as in iterative 2D/3D registration."""
+    R = np.eye(3)
+    for ax, a in enumerate(np.deg2rad(rng.uniform(-5, 5, 3))):
+        c, s = np.cos(a), np.sin(a)
+        M = np.eye(3)
+        i, j = [(1, 2), (0, 2), (0, 1)][ax]
+        M[i, i] = c
+        M[j, j] = c
+        M[i, j] = -s
+        M[j, i] = s
+        R = R @ M
+    rot = R @ _FIXED
+    pos = rot @ np.array([0, 0, -sid]) + rng.uniform(-10, 10, 3)
+    setup = AcquisitionSetup(CTSystem(CarmGantry(span), FlatPanelDetector((n, n), (pitch_mm, pitch_mm))))
+    setup.setCarmPoses(pos[None], rot.reshape(1, 9))
+    return GeometryEncoder.encodeFullGeometry(setup)
+
+
+def served_scan_geometry(views: int = 180, n: int = 768, pitch_mm: float = 0.533) -> np.ndarray:
+    """One full scan of the served workload: a 180-view circle on 768 x 768 pixels."""
+    return carm_circle(views, (n, n), (pitch_mm, pitch_mm))
+
+
+def served_phantoms(count: int = 8, n: int = 256, voxel_mm: float = 0.5) -> list:
+    """The served workload's resident phantoms (20 ellipsoids each, seeds 5000 + k)."""
+    return [ellipsoid_phantom((n, n, n), (voxel_mm,) * 3, 20, 5000 + k) for k in range(count)]

@@ -139,7 +139,7 @@ ctlproj_status ctlproj_project_device(ctlproj_projector* p, const ctlproj_volume
 /* Allocation-free form of ctlproj_project_device. No counterpart in the reference: it is for
  * callers that cannot allocate device memory during a projection (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree: the
- * per-(view, module) ray geometry goes into a static device scratch of 8 MiB per device, held
+ * per-(view, module) ray geometry goes into a static device scratch of 20 MiB per device, held
  * until the process ends, which limits a projection to CTLPROJ_NOALLOC_MAX_VIEW_MODULES
  * (view, module) pairs (larger ones return CTLPROJ_ERR_INVALID_ARGUMENT, as does a volume
  * upsampling factor other than 1, which would need a temporary volume). Same kernels and results
@@ -246,6 +246,11 @@ ctlproj_status ctlproj_batch_last_ms(ctlproj_batch* b, float* ms);
 ctlproj_status ctlproj_batch_info(const ctlproj_batch* b, uint32_t* tile_w, uint32_t* tile_h,
                                   uint32_t* unroll, uint32_t* grid, uint64_t* tiles);
 ctlproj_status ctlproj_batch_count_samples(ctlproj_batch* b, uint64_t* n_samples);
+/* The finalized batch's launch shape as the tuning fields of *settings (tile, unroll, workgroups
+ * per CU, LDS chunk (-1: unstaged) and box); the other fields are left as they are. A single
+ * projection with these fields runs the same per-pixel code as the batch and gives the same
+ * bits. */
+ctlproj_status ctlproj_batch_shape(const ctlproj_batch* b, ctlproj_settings* settings);
 
 /* ---- serving (no counterpart in the reference) ---------------------------------------------
  * A request loop over resident volumes: requests are admitted when their device footprint

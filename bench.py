@@ -20,9 +20,16 @@ import time
 import numpy as np
 
 HBM_PEAK_GBPS = 8000.0
-# Vector L1 to register data path of gfx950 for 16-byte loads: 256 CUs x 2.4 GHz x 64 B/clk. The
-# stream probe (coalesced 16-byte loads of an L1-resident window) reaches 98% of it.
+# On-chip data paths the staged ray caster gathers through, per CU per clock on gfx950, times
+# 256 CUs x 2.4 GHz:
+#   vector L1 -> registers, 16-byte loads: 64 B/clk (39.3 TB/s); the stream probe (coalesced
+#     16-byte loads of an L1-resident window) reaches 98% of it;
+#   LDS -> registers, ds_read_b128: 256 B/clk (157.3 TB/s); the LDS probe (conflict-free 16-byte
+#     reads) reaches 85-90% of it. Two adjacent 8-byte reads per lane (ds_read2_b64, the way the
+#     ray caster reads its texel pairs) reach only a quarter of that (the lds_pairs probe).
 L1_PEAK_GBPS = 256 * 2.4 * 64.0
+LDS_PEAK_GBPS = 256 * 2.4 * 256.0
+ONCHIP_PEAK_GBPS = L1_PEAK_GBPS + LDS_PEAK_GBPS
 
 # HBM traffic per launch of the 360-view FP32 projection (LDS-staged kernel), from the rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_kernel_f32_256cube_360views_1024x768.md, in
@@ -31,8 +38,6 @@ L1_PEAK_GBPS = 256 * 2.4 * 64.0
 FETCH_KIB, FETCH_CAL = 5.1107e7, 1048576 / 524300
 WRITE_KIB, WRITE_CAL = 1.1767e6, 1.0
 TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
-# LDS read bandwidth of gfx950: 256 CUs x 2.4 GHz x 128 B/clk.
-LDS_PEAK_GBPS = 256 * 2.4 * 128.0
 
 
 def log(msg: str) -> None:
@@ -85,6 +90,9 @@ def bench_real(args, torch, W, ctl, oracle):
         if storage == "f32":
             ps = [p.gather_probe(vol, "stream", out.data_ptr(), out.numel()) for _ in range(4)][1:]
             res[storage]["probe_l1_stream_GBps"] = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in ps]))
+            for mode in ("lds", "lds_pairs"):
+                pl = [p.gather_probe(vol, mode, out.data_ptr(), out.numel()) for _ in range(4)][1:]
+                res[storage][f"probe_{mode}_GBps"] = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in pl]))
         log(f"real {storage}: {json.dumps(res[storage])}")
         del vol
     if args.kernels_only:
@@ -445,24 +453,28 @@ def main() -> int:
                    "views": 360, "detector": "1024x768 @ 0.4 mm, C-arm 1200 mm / SID 750 mm",
                    "ray_step_vox": 0.2, "parallelism": "single GPU"},
         "views_per_s": f32["views_s"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": TRAFFIC_F32_BYTES,
+        "roofline": {"bound": "lds+l1", "achieved": achieved, "peak": ONCHIP_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / ONCHIP_PEAK_GBPS, "traffic": TRAFFIC_F32_BYTES,
                      "algorithmic_bytes_per_sample": 32, "samples_per_launch": f32["n_samples"],
                      "kernel_ms": f32["kernel_ms"],
-                     "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample). The "
-                             "staged kernel serves them from LDS boxes filled once per 64-sample "
-                             "chunk; traffic = calibrated HBM/fabric bytes per launch. On-chip "
-                             "peaks: LDS 128 B/clk/CU (lds_peak), vector L1 64 B/clk/CU "
-                             "(l1_gather_peak, the ceiling of gathering without LDS). The staged "
-                             "kernel is bound on chip by three pipes together: 33 VALU "
-                             "instructions per sample (1.36 per CU per clock), the LDS pipe (0.70 "
-                             "busy with bank conflicts) and the texture data path (0.93 busy: a "
-                             "third of the samples gather through L1)",
-                     "lds_peak": LDS_PEAK_GBPS, "lds_frac": achieved / LDS_PEAK_GBPS,
-                     "l1_gather_peak": L1_PEAK_GBPS, "l1_gather_frac": achieved / L1_PEAK_GBPS,
+                     "note": "achieved = algorithmic gather bytes (8 taps x 4 B per sample) / kernel "
+                             "time; peak = the on-chip data paths the staged kernel gathers through "
+                             "together: LDS (ds_read_b128, 256 B/clk/CU) + vector L1 (64 B/clk/CU), "
+                             "256 CUs x 2.4 GHz, each checked by a probe (probe_lds_GBps, "
+                             "probe_l1_stream_GBps). traffic = calibrated HBM bytes per launch "
+                             "(hbm_GBps, hbm_frac). The kernel reads its texel pairs with "
+                             "ds_read2_b64, which the lds_pairs probe measures at a quarter of the "
+                             "b128 rate; it is bound by latency and issue together (VALU, the LDS "
+                             "pair reads and the texture path with the third of the samples it "
+                             "gathers through L1), not by any one data path",
+                     "lds_peak": LDS_PEAK_GBPS, "l1_peak": L1_PEAK_GBPS,
+                     "probe_lds_GBps": f32.get("probe_lds_GBps"),
+                     "probe_lds_pairs_GBps": f32.get("probe_lds_pairs_GBps"),
+                     "probe_l1_stream_GBps": f32.get("probe_l1_stream_GBps"),
                      "probe_gather_GBps": f32.get("probe_gather_GBps"),
                      "frac_of_probe_gather": f32.get("frac_of_probe_gather"),
-                     "probe_l1_stream_GBps": f32.get("probe_l1_stream_GBps")},
+                     "hbm_GBps": TRAFFIC_F32_BYTES / (f32["kernel_ms"] * 1e-3) / 1e9,
+                     "hbm_frac": TRAFFIC_F32_BYTES / (f32["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS},
         "cpu_baseline": cpu,
         "variants": real,
     }

@@ -972,7 +972,8 @@ ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_vol
                                          float* device_out, size_t out_count, float* ms, double* bytes)
 {
     if (!ms || !bytes) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (mode != CTLPROJ_PROBE_GATHER && mode != CTLPROJ_PROBE_STREAM)
+    if (mode != CTLPROJ_PROBE_GATHER && mode != CTLPROJ_PROBE_STREAM && mode != CTLPROJ_PROBE_LDS &&
+        mode != CTLPROJ_PROBE_LDS_PAIRS)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "unknown probe mode");
     ctlproj_status st = CTLPROJ_OK;
     hipError_t e = hipSuccess;
@@ -1010,15 +1011,20 @@ ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_vol
     if (e != hipSuccess) return hipFail(e, "hipDeviceGetAttribute");
     const uint32_t grid = uint32_t(cus) * 8, iters = 16384;
     if (out_count < size_t(grid) * 256)
-        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream probe needs 2048 floats per CU of output");
-    if (vol->view.bytes < 16384) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream probe needs a volume of 16 KiB or more");
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream and LDS probes need 2048 floats per CU of output");
+    if (mode == CTLPROJ_PROBE_STREAM && vol->view.bytes < 16384)
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the stream probe needs a volume of 16 KiB or more");
     e = hipEventRecord(p->ev_start, p->stream);
-    if (e == hipSuccess) e = ctlk::launchProbeStream(vol->view.data, grid, iters, device_out, p->stream);
+    if (e == hipSuccess)
+        e = mode == CTLPROJ_PROBE_STREAM
+                ? ctlk::launchProbeStream(vol->view.data, grid, iters, device_out, p->stream)
+                : ctlk::launchProbeLds(grid, iters, mode == CTLPROJ_PROBE_LDS_PAIRS, device_out, p->stream);
     if (e == hipSuccess) e = hipEventRecord(p->ev_stop, p->stream);
     if (e == hipSuccess) e = hipEventSynchronize(p->ev_stop);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, p->ev_start, p->ev_stop);
-    if (e != hipSuccess) return hipFail(e, "stream probe");
-    *bytes = double(grid) * 256.0 * iters * 16.0;
+    if (e != hipSuccess) return hipFail(e, mode == CTLPROJ_PROBE_STREAM ? "stream probe" : "LDS probe");
+    // the pair probe reads 15 of every 16 elements
+    *bytes = double(grid) * 256.0 * iters * 16.0 * (mode == CTLPROJ_PROBE_LDS_PAIRS ? 15.0 / 16.0 : 1.0);
     return CTLPROJ_OK;
 }
 

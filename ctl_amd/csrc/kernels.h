@@ -217,6 +217,10 @@ hipError_t launchZero(float* d, size_t n, hipStream_t s);
 hipError_t launchProbeGather(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                              const RayParams& rp, float* d_out, hipStream_t s);
 hipError_t launchProbeStream(const void* d_src, uint32_t grid, uint32_t iters, float* d_out, hipStream_t s);
+// LDS read ceiling: `grid` workgroups of conflict-free 16-byte LDS reads (pairs: as two adjacent
+// 8-byte elements, 15 of every 16 reads), `iters` (a multiple of 16) per thread, from a 16 KiB LDS
+// window; d_out receives grid * 256 floats.
+hipError_t launchProbeLds(uint32_t grid, uint32_t iters, bool pairs, float* d_out, hipStream_t s);
 
 // Mean of k contiguous projections of n values (intensity != 0: -ln of the mean of exp(-value)).
 hipError_t launchAverage(const float* d_in, uint32_t k, size_t n, int intensity, float* d_out, hipStream_t s);

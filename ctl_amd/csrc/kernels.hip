@@ -963,6 +963,44 @@ __global__ __launch_bounds__(kBlock) void probe_stream_kernel(const void* __rest
     out[size_t(blockIdx.x) * kBlock + threadIdx.x] = float(acc & 0xffffu);
 }
 
+// LDS read ceiling: each workgroup fills a 16 KiB LDS window, then every lane reads 16-byte
+// elements, a wavefront 1 KiB contiguous per ds_read_b128 (no bank conflict), 16 reads per step at
+// immediate offsets from a base that moves by a runtime `step` (0 here, so the compiler cannot
+// hoist the reads out of the loop). Two three-operand adds per read keep the values live.
+// PAIRS = true reads the same 16 bytes per lane as two adjacent 8-byte elements at an odd 8-byte
+// index (ds_read2_b64 ... offset1:1), the way the staged ray caster reads a texel pair from its box.
+template <bool PAIRS>
+__global__ __launch_bounds__(kBlock) void probe_lds_kernel(uint32_t iters, uint32_t step, float* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) probe_u4 win[1024]; // 16 KiB
+    for (uint32_t i = threadIdx.x; i < 1024; i += kBlock) win[i] = probe_u4{ i, i * 3u, i * 5u, i * 7u };
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t acc0 = 0, acc1 = 0, base = 0;
+    for (uint32_t k = 0; k < iters; k += 16) {
+        if constexpr (PAIRS) {
+            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+            const u2* w = reinterpret_cast<const u2*>(win) + ((base + 2 * lane + 1) & 2047u);
+#pragma unroll
+            for (uint32_t u = 0; u < 15; ++u) { // 15 x 1 KiB: element 2047 is the last one read
+                const u2 a = w[u * 128], b = w[u * 128 + 1];
+                acc0 += a.x + a.y;
+                acc1 += b.x + b.y;
+            }
+        } else {
+            const probe_u4* w = win + ((base + lane) & 1023u);
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) {
+                const probe_u4 a = w[u * 64];
+                acc0 += a.x + a.y;
+                acc1 += a.z + a.w;
+            }
+        }
+        base += step;
+    }
+    out[size_t(blockIdx.x) * kBlock + threadIdx.x] = float((acc0 + acc1) & 0xffffu);
+}
+
 // ---------------------------------------------------------------------------------------------
 // volume conversion: every padded texel (x, y, z) is written from a value function v(x, y, z)
 // of the padded frame that returns 0 outside the interior.
@@ -1257,6 +1295,14 @@ hipError_t launchCombineViews(const float* d_a, const float* d_ca, const float* 
     const uint64_t pb = (uint64_t(n) + kBlock - 1) / kBlock, blocks = pb * views;
     if (blocks > 0x7fffffffull) return hipErrorInvalidConfiguration;
     combine_views_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(d_a, d_ca, d_b, d_cb, n, uint32_t(pb), d_out);
+    return hipGetLastError();
+}
+
+hipError_t launchProbeLds(uint32_t grid, uint32_t iters, bool pairs, float* d_out, hipStream_t s)
+{
+    if (grid == 0 || iters % 16) return hipErrorInvalidValue;
+    if (pairs) probe_lds_kernel<true><<<grid, kBlock, 0, s>>>(iters, 0u, d_out);
+    else probe_lds_kernel<false><<<grid, kBlock, 0, s>>>(iters, 0u, d_out);
     return hipGetLastError();
 }
 

@@ -570,9 +570,11 @@ class RayCasterProjector:
     def gather_probe(self, volume: ResidentVolume, mode: str, device_out: int, out_count: int):
         """Gather-ceiling diagnostic (ctlproj_diag_gather_probe): mode "gather" issues this
         projection's texel loads without the interpolation, mode "stream" reads a 16 KiB
-        L1-resident window with coalesced 16-byte loads. Returns (milliseconds, bytes loaded)."""
+        L1-resident window with coalesced 16-byte loads, mode "lds" reads a 16 KiB LDS window
+        with conflict-free 16-byte reads, mode "lds_pairs" the same bytes as two adjacent 8-byte
+        reads per lane. Returns (milliseconds, bytes loaded)."""
         self._push_settings()
-        code = {"gather": 0, "stream": 1}[mode]
+        code = {"gather": 0, "stream": 1, "lds": 2, "lds_pairs": 3}[mode]
         ms, nbytes = C.c_float(), C.c_double()
         check(lib().ctlproj_diag_gather_probe(self.h, volume.h, code, C.c_void_p(device_out),
                                               out_count, C.byref(ms), C.byref(nbytes)))

@@ -517,8 +517,18 @@ ctlproj_status ctlproj_radon3d_sample_transform(const ctlproj_volume* vol, const
  *                         *bytes = samples x 8 taps x element size.
  *   CTLPROJ_PROBE_STREAM  coalesced 16-byte loads from a 16 KiB window of the volume that stays in
  *                         the vector L1; device_out needs 2048 floats per CU. *bytes = bytes loaded.
+ *   CTLPROJ_PROBE_LDS     conflict-free 16-byte LDS reads (ds_read_b128, a wavefront 1 KiB
+ *                         contiguous) from a 16 KiB LDS window per workgroup; the volume is not
+ *                         read; device_out needs 2048 floats per CU. *bytes = bytes read from LDS.
+ *   CTLPROJ_PROBE_LDS_PAIRS  the same 16 bytes per lane read as two adjacent 8-byte elements
+ *                         (ds_read2_b64), the instruction the staged ray caster reads texel pairs with.
  * *ms is the probe kernel's duration (HIP events on the projector's stream). */
-typedef enum ctlproj_probe_mode { CTLPROJ_PROBE_GATHER = 0, CTLPROJ_PROBE_STREAM = 1 } ctlproj_probe_mode;
+typedef enum ctlproj_probe_mode {
+    CTLPROJ_PROBE_GATHER = 0,
+    CTLPROJ_PROBE_STREAM = 1,
+    CTLPROJ_PROBE_LDS = 2,
+    CTLPROJ_PROBE_LDS_PAIRS = 3
+} ctlproj_probe_mode;
 ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_volume* vol, int mode,
                                          float* device_out, size_t out_count, float* ms, double* bytes);
 

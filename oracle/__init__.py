@@ -56,6 +56,10 @@ def lib():
         L.oracle_sample_linear64.argtypes = [f, u, C.c_double, C.c_double, C.c_double]
         L.oracle_sample_linear64.restype = C.c_double
         L.oracle_clip32.argtypes = [f, f, f, f, C.c_int, f]
+        L.oracle_cpu_interpolated_read.argtypes = [f, u, C.c_double, C.c_double, C.c_double]
+        L.oracle_cpu_interpolated_read.restype = C.c_float
+        L.oracle_cpu_nearest_read.argtypes = [f, u, C.c_double, C.c_double, C.c_double]
+        L.oracle_cpu_nearest_read.restype = C.c_float
         L.oracle_slice.argtypes = [f, u, f, f, d, C.c_double, C.c_uint32, C.c_uint32, C.c_float,
                                    C.c_int, f]
         L.oracle_slice.restype = C.c_int
@@ -119,6 +123,15 @@ def sample_linear64(vol, px, py, pz):
     z, y, x = v.shape
     dims = (C.c_uint32 * 3)(x, y, z)
     return lib().oracle_sample_linear64(_f(v), dims, px, py, pz)
+
+
+def cpu_read(vol, px, py, pz, interpolate=True):
+    """RayCasterProjectorCPU's interpolatedRead / nonInterpolatedRead at an FP64 position."""
+    v = np.ascontiguousarray(vol, np.float32)
+    z, y, x = v.shape
+    dims = (C.c_uint32 * 3)(x, y, z)
+    fn = lib().oracle_cpu_interpolated_read if interpolate else lib().oracle_cpu_nearest_read
+    return fn(_f(v), dims, px, py, pz)
 
 
 def _vol_args(vol):
@@ -185,8 +198,9 @@ def clip32(source, direction, vol_size, vol_corner, interpolate=True):
 
 def project(volume, voxel_mm, offset_mm, pmats, channels, rows, ray_sampling=0.3,
             rays_per_pixel=(1, 1), upsampling=1, interpolate=True, fp64=False, threads=None,
-            views=None, want64=False):
-    """Runs ref32 (fp64=False) or ref64 (fp64=True). volume: (Z, Y, X) float32;
+            views=None, want64=False, cpu_projector=False):
+    """Runs ref32 (fp64=False) or ref64 (fp64=True), or with cpu_projector=True the restatement
+    of RayCasterProjectorCPU (refcpu; no upsampling). volume: (Z, Y, X) float32;
     pmats: (V, M, 3, 4). Returns (out[V', M, R, C] float32, n_samples[, out64])."""
     vol = np.ascontiguousarray(volume, np.float32)
     z, y, x = vol.shape
@@ -205,12 +219,12 @@ def project(volume, voxel_mm, offset_mm, pmats, channels, rows, ray_sampling=0.3
     else:
         vptr, nv = None, V
     out = np.zeros((nv, M, rows, channels), np.float32)
-    out64 = np.zeros(out.shape, np.float64) if (want64 and fp64) else None
+    out64 = np.zeros(out.shape, np.float64) if (want64 and fp64 and not cpu_projector) else None
     n = C.c_uint64()
     threads = threads or os.cpu_count() or 1
     rc = lib().oracle_project(_f(vol), (C.c_uint32 * 3)(x, y, z), (C.c_float * 3)(*voxel_mm),
                               (C.c_float * 3)(*offset_mm), _d(pm), V, M, channels, rows,
-                              C.byref(s), int(fp64), int(threads), vptr, nv, _f(out),
+                              C.byref(s), 2 if cpu_projector else int(bool(fp64)), int(threads), vptr, nv, _f(out),
                               _d(out64) if out64 is not None else None, C.byref(n))
     if rc != 0:
         raise ValueError(f"oracle_project failed ({rc})")

@@ -318,6 +318,58 @@ static void clip64(const double s[3], const double d[3], const double vol_size[3
 }
 
 /* ------------------------------------------------------------------------------------------- */
+/* refcpu: RayCasterProjectorCPU (modules/src/projectors/raycasterprojectorcpu.cpp)              */
+/* ------------------------------------------------------------------------------------------- */
+
+/* VoxelVolume<float>::operator() (voxelvolume.tpp:324-332): the linear index z*X*Y + y*X + x
+ * without a per-axis bounds check. The reference asserts lup < size in debug builds and reads
+ * past the array in release builds; an index beyond the array reads 0 here. */
+static float voxel_linear(const float* v, const uint32_t* n, int x, int y, int z)
+{
+    const size_t lup = (size_t)(uint32_t)z * n[0] * n[1] + (size_t)(uint32_t)y * n[0] + (size_t)(uint32_t)x;
+    return lup < (size_t)n[0] * n[1] * n[2] ? v[lup] : 0.0f;
+}
+
+/* interpolatedRead (raycasterprojectorcpu.cpp:239-285): FP64 position, FP32 weights and values,
+ * taps at or beyond a low border read 0, taps beyond the high border read 0 except the v0xx taps
+ * of a position exactly at n + 1/2 on that axis, where floor(p - 1/2) = n is not masked and the
+ * read lands on the linear index (the reference's out-of-bounds read). */
+float oracle_cpu_interpolated_read(const float* v, const uint32_t n[3], double px, double py, double pz)
+{
+    if (px < -0.5 || py < -0.5 || pz < -0.5 || px > n[0] + 0.5 || py > n[1] + 0.5 || pz > n[2] + 0.5)
+        return 0.0f;
+    const int vx = (int)floor(px - 0.5), vy = (int)floor(py - 0.5), vz = (int)floor(pz - 0.5);
+    const int lo0 = vx < 0, lo1 = vy < 0, lo2 = vz < 0;
+    const int hi0 = vx >= (int)n[0] - 1, hi1 = vy >= (int)n[1] - 1, hi2 = vz >= (int)n[2] - 1;
+    const float w0 = (float)(px - (vx + 0.5)), w1 = (float)(py - (vy + 0.5)), w2 = (float)(pz - (vz + 0.5));
+    const float v000 = (lo0 || lo1 || lo2) ? 0.0f : voxel_linear(v, n, vx, vy, vz);
+    const float v001 = (lo0 || lo1 || hi2) ? 0.0f : voxel_linear(v, n, vx, vy, vz + 1);
+    const float v010 = (lo0 || hi1 || lo2) ? 0.0f : voxel_linear(v, n, vx, vy + 1, vz);
+    const float v011 = (lo0 || hi1 || hi2) ? 0.0f : voxel_linear(v, n, vx, vy + 1, vz + 1);
+    const float v100 = (hi0 || lo1 || lo2) ? 0.0f : voxel_linear(v, n, vx + 1, vy, vz);
+    const float v101 = (hi0 || lo1 || hi2) ? 0.0f : voxel_linear(v, n, vx + 1, vy, vz + 1);
+    const float v110 = (hi0 || hi1 || lo2) ? 0.0f : voxel_linear(v, n, vx + 1, vy + 1, vz);
+    const float v111 = (hi0 || hi1 || hi2) ? 0.0f : voxel_linear(v, n, vx + 1, vy + 1, vz + 1);
+    const float w0o = 1.0f - w0;
+    const float c00 = w0o * v000 + w0 * v100;
+    const float c01 = w0o * v001 + w0 * v101;
+    const float c10 = w0o * v010 + w0 * v110;
+    const float c11 = w0o * v011 + w0 * v111;
+    const float w1o = 1.0f - w1;
+    const float c0 = c00 * w1o + c10 * w1;
+    const float c1 = c01 * w1o + c11 * w1;
+    return c0 * (1.0f - w2) + c1 * w2;
+}
+
+/* nonInterpolatedRead (raycasterprojectorcpu.cpp:287-299): floor of the FP64 position; a
+ * position exactly at n on an axis passes the range test and reads the linear index. */
+float oracle_cpu_nearest_read(const float* v, const uint32_t n[3], double px, double py, double pz)
+{
+    if (px < 0.0 || py < 0.0 || pz < 0.0 || px > n[0] || py > n[1] || pz > n[2]) return 0.0f;
+    return voxel_linear(v, n, (int)floor(px), (int)floor(py), (int)floor(pz));
+}
+
+/* ------------------------------------------------------------------------------------------- */
 /* projection driver                                                                            */
 /* ------------------------------------------------------------------------------------------- */
 
@@ -336,6 +388,7 @@ typedef struct job {
     uint32_t n_out;
     oracle_settings s;
     int fp64;
+    int cpu;  /* 1: RayCasterProjectorCPU restatement (refcpu) */
     float* out;
     double* out64;
     /* work distribution: one task = one (output view, module, row) */
@@ -344,6 +397,56 @@ typedef struct job {
     pthread_mutex_t lock;
     uint64_t samples;
 } job;
+
+/* One detector row of RayCasterProjectorCPU::computeView (raycasterprojectorcpu.cpp:132-235):
+ * FP64 direction from the packed QR normalised in FP64, scaled per axis by the FP32 increment in
+ * voxels (increment_mm / voxel size, both FP32); FP64 source, corner (FP32 volumeCorner / voxel
+ * size) and slab clip; positions std::fma(i, direction, corner-to-source) in FP64; FP32
+ * interpolation; FP64 sum; out = increment_mm * (float)sum / (float)(rx * ry). */
+static void run_row_cpu(const job* J, const double qr[16], const double src64[3], size_t out_base,
+                        uint32_t y, uint64_t* samples)
+{
+    const uint32_t rx = J->s.rays_per_pixel[0], ry = J->s.rays_per_pixel[1];
+    const int interp = J->s.interpolate;
+    double source[3], corner[3], c2s[3], vsize[3], inc_vox[3];
+    for (int k = 0; k < 3; ++k) {
+        source[k] = src64[k] / (double)J->vs[k];
+        corner[k] = (double)J->corner[k] / (double)J->vs[k];
+        inc_vox[k] = (double)(J->increment / J->vs[k]);
+        c2s[k] = source[k] - corner[k];
+        vsize[k] = (double)J->n[k];
+    }
+    const double ipx = 1.0 / (double)rx, ipy = 1.0 / (double)ry;
+    for (uint32_t x = 0; x < J->channels; ++x) {
+        const double pcx = ((double)x - 0.5) + 0.5 * ipx;
+        const double pcy = ((double)y - 0.5) + 0.5 * ipy;
+        double acc = 0.0;
+        for (uint32_t a = 0; a < rx; ++a)
+            for (uint32_t b = 0; b < ry; ++b) {
+                double d[3];
+                oracle_direction64(qr, pcx + (double)a * ipx, pcy + (double)b * ipy, d);
+                const double len = sqrt(0.0 + d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                for (int k = 0; k < 3; ++k) d[k] = d[k] / len * inc_vox[k];
+                double mm[2];
+                clip64(source, d, vsize, corner, interp, mm);
+                /* a miss (DBL_MAX) contributes nothing, as in ref32: the reference's (uint) cast
+                 * of DBL_MAX is undefined and, on x86, samples two positions next to the source */
+                if (mm[0] == DBL_MAX) continue;
+                const uint32_t i0 = (uint32_t)mm[0];
+                const uint32_t i1 = (uint32_t)mm[1] + 1u;
+                for (uint32_t i = i0; i <= i1; ++i) {
+                    const double fi = (double)i;
+                    const double p0 = fma(fi, d[0], c2s[0]);
+                    const double p1 = fma(fi, d[1], c2s[1]);
+                    const double p2 = fma(fi, d[2], c2s[2]);
+                    acc += (double)(interp ? oracle_cpu_interpolated_read(J->vol, J->n, p0, p1, p2)
+                                           : oracle_cpu_nearest_read(J->vol, J->n, p0, p1, p2));
+                }
+                if (i1 >= i0) *samples += (uint64_t)(i1 - i0) + 1u;
+            }
+        J->out[out_base + x] = J->increment * (float)acc / (float)(rx * ry);
+    }
+}
 
 static void run_row(job* J, uint32_t ov, uint32_t m, uint32_t y, uint64_t* samples)
 {
@@ -357,6 +460,10 @@ static void run_row(job* J, uint32_t ov, uint32_t m, uint32_t y, uint64_t* sampl
     const size_t out_base = (((size_t)ov * J->n_modules + m) * J->rows + y) * J->channels;
     const int interp = J->s.interpolate;
 
+    if (J->cpu) {
+        run_row_cpu(J, qr, src64, out_base, y, samples);
+        return;
+    }
     if (!J->fp64) {
         /* kernel prologue, raycasterprojector_interp.cl:35-40 */
         const float src_mm[3] = { (float)src64[0], (float)src64[1], (float)src64[2] };
@@ -480,8 +587,9 @@ int oracle_project(const float* volume, const uint32_t dims[3], const float voxe
     job J;
     memset(&J, 0, sizeof J);
     J.s = *settings;
-    /* non-interpolating mode forces the upsampling factor to 1 (raycasterprojector.cpp:148-151) */
-    const uint32_t f = J.s.interpolate ? J.s.volume_upsampling : 1u;
+    /* non-interpolating mode forces the upsampling factor to 1 (raycasterprojector.cpp:148-151);
+     * RayCasterProjectorCPU has no upsampling setting */
+    const uint32_t f = (J.s.interpolate && fp64 != 2) ? J.s.volume_upsampling : 1u;
 
     /* VolumeSpecs::upSampleVolume (raycasterprojector.cpp:576-672) */
     float* up = NULL;
@@ -528,7 +636,8 @@ int oracle_project(const float* volume, const uint32_t dims[3], const float voxe
     J.rows = rows;
     J.views = view_subset;
     J.n_out = view_subset ? n_subset : n_views;
-    J.fp64 = fp64;
+    J.fp64 = fp64 == 1;
+    J.cpu = fp64 == 2;
     J.out = out;
     J.out64 = out64;
     J.n_tasks = (uint64_t)J.n_out * n_modules * rows;

@@ -11,7 +11,11 @@
  *             raycasterprojector_no_interp.cl:16-154, host side
  *             modules/src/projectors/raycasterprojector.cpp:139-375, 527-672);
  *   ref64 -- the same sample positions and loop bounds with every quantity in FP64
- *            (the "FP64 CPU ray-caster" that error statements are made against).
+ *            (the "FP64 CPU ray-caster" that error statements are made against);
+ * and the reference's own CPU projector:
+ *   refcpu -- RayCasterProjectorCPU (modules/src/projectors/raycasterprojectorcpu.cpp:132-407):
+ *            FP64 direction, positions (std::fma), clip and sum; FP32 weights and values; its
+ *            unmasked read at a position exactly n + 1/2 (n for nearest) on an axis.
  * Plus the FP64 geometry helpers the host side uses (QR of M, source position).
  */
 #ifndef CTL_ORACLE_H
@@ -53,7 +57,8 @@ void oracle_direction64(const double qr16[16], double x, double y, double d[3]);
  *   pmats    [n_views][n_modules][3][4] row-major FP64 projection matrices
  *   out      [n_out][n_modules][rows][channels] float, n_out = n_subset or n_views
  *   out64    optional FP64 copy of the same values (ref64 only; may be NULL)
- *   fp64     0 -> ref32, 1 -> ref64
+ *   fp64     0 -> ref32, 1 -> ref64, 2 -> refcpu (rays per pixel and interpolation apply, the
+ *            upsampling factor does not: the CPU projector has none)
  *   n_samples  optional: total number of sampling-loop iterations (reference loop count)
  * Returns 0 on success, nonzero on invalid arguments.
  */
@@ -68,6 +73,9 @@ int oracle_project(const float* volume, const uint32_t dims[3], const float voxe
 float oracle_sample_linear32(const float* vol, const uint32_t dims[3], float px, float py, float pz);
 double oracle_sample_linear64(const float* vol, const uint32_t dims[3], double px, double py,
                               double pz);
+/* refcpu single-sample reads (interpolatedRead / nonInterpolatedRead). */
+float oracle_cpu_interpolated_read(const float* vol, const uint32_t dims[3], double px, double py, double pz);
+float oracle_cpu_nearest_read(const float* vol, const uint32_t dims[3], double px, double py, double pz);
 /* ref32 slab clip (raycasterprojector_interp.cl:88-148); writes (lambda_min, lambda_max). */
 void oracle_clip32(const float source[3], const float direction[3], const float vol_size[3],
                    const float vol_corner[3], int interpolate, float minmax[2]);

@@ -37,6 +37,64 @@ def test_project_composite_is_the_sum_of_the_projections():
     assert p.isLinear()
 
 
+def test_project_composite_against_oracle_sum():
+    """projectComposite (abstractprojector.h:99-112) of three sub-volumes with their own sizes,
+    voxel sizes and offsets (an FP32 phantom, an FP16 insert of 0.8 mm voxels off centre, an FP32
+    slab of anisotropic voxels) against the sum of the oracle's projections of each: ref32 for the
+    FP32 parts, ref64 for the FP16 insert (FP16 storage is measured against ref64), summed in FP64."""
+    w = W.small(views=36)
+    pm = w.pmats[::6]
+    a = np.ascontiguousarray(w.volume)
+    b = np.ascontiguousarray(W.ellipsoid_phantom((40, 36, 30), (0.8, 0.8, 0.8), 6, 91, semi_axes=(2.0, 8.0)))
+    c = np.ascontiguousarray(W.ellipsoid_phantom((48, 20, 52), (1.1, 1.5, 0.9), 5, 92, semi_axes=(3.0, 9.0)))
+    parts = [(a, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), "f32"), (b, (0.8, 0.8, 0.8), (4.0, -3.0, 2.5), "f16"),
+             (c, (1.1, 1.5, 0.9), (-6.0, 2.0, -1.5), "f32")]
+    p = RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((pm, w.channels, w.rows))
+    comp = p.projectComposite([ResidentVolume(VoxelVolume(v, vs, off), st) for v, vs, off, st in parts]).data
+    ref = np.zeros(comp.shape)
+    for v, vs, off, st in parts:
+        r, _ = oracle.project(v, vs, off, pm, w.channels, w.rows, w.ray_sampling, fp64=(st != "f32"))
+        ref += r
+    e, m = oracle.rel_l2(ref, comp), oracle.max_abs_rel(ref, comp)
+    print(f"composite of 3 sub-volumes vs oracle sum: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= 5e-4  # the FP16 part's storage rounding; the FP32 parts alone stay below 1e-4
+    fp32_only = p.projectComposite([ResidentVolume(VoxelVolume(v, vs, off), st) for v, vs, off, st in parts
+                                    if st == "f32"]).data
+    ref32 = sum(oracle.project(v, vs, off, pm, w.channels, w.rows, w.ray_sampling)[0].astype(np.float64)
+                for v, vs, off, st in parts if st == "f32")
+    e32, m32 = oracle.rel_l2(ref32, fp32_only), oracle.max_abs_rel(ref32, fp32_only)
+    print(f"composite of the FP32 sub-volumes vs ref32 sum: rel L2 {e32:.3e}, max abs rel {m32:.3e}")
+    assert e32 <= 1e-4 and m32 <= 1e-3
+
+
+def test_fp32_drop_in_against_the_reference_cpu_projector():
+    """The FP32 drop-in against refcpu, the restatement of the reference's RayCasterProjectorCPU
+    (raycasterprojectorcpu.cpp:132-285: FP64 positions, FP32 weights), under the drop-in bars, on
+    the small workload and on two views of the 256^3 / 1024 x 768 one."""
+    w = W.small(views=36)
+    rc, _ = oracle.project(w.volume, w.voxel_mm, w.offset_mm, w.pmats, w.channels, w.rows, w.ray_sampling,
+                           cpu_projector=True)
+    p = RayCasterProjector()
+    p.settings().raySampling = w.ray_sampling
+    p.configure((w.pmats, w.channels, w.rows))
+    got = p.project(ResidentVolume(VoxelVolume(w.volume, w.voxel_mm, w.offset_mm))).data
+    e, m = oracle.rel_l2(rc, got), oracle.max_abs_rel(rc, got)
+    print(f"small f32 vs refcpu: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= 1e-4 and m <= 1e-3
+    real = W.real(views=360)
+    pick = [0, 137]
+    rc, _ = oracle.project(real.volume, real.voxel_mm, real.offset_mm, real.pmats, real.channels, real.rows,
+                           real.ray_sampling, views=pick, threads=16, cpu_projector=True)
+    p.settings().raySampling = real.ray_sampling
+    p.configure((real.pmats[pick], real.channels, real.rows))
+    got = p.project(ResidentVolume(VoxelVolume(real.volume, real.voxel_mm, real.offset_mm))).data
+    e, m = oracle.rel_l2(rc, got), oracle.max_abs_rel(rc, got)
+    print(f"256^3 views {pick} f32 vs refcpu: rel L2 {e:.3e}, max abs rel {m:.3e}")
+    assert e <= 1e-4 and m <= 1e-3
+
+
 @pytest.mark.parametrize("low_extinction", [False, True])
 def test_areal_focal_spot_against_oracle(low_extinction):
     w = W.small(views=4)

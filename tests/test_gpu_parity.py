@@ -123,7 +123,7 @@ def check_same_samples(plain, got, what):
 @pytest.mark.parametrize("chunk,texels", [(16, 0), (64, 0), (256, 0), (64, 1024), (1024, 20352)])
 def test_lds_staged_matches_unstaged_and_ref32(small, small_refs, chunk, texels):
     """FP32 trilinear projections stage the taps of each chunk of samples in LDS; 1024 texels
-    forces the HBM fallback for the larger boxes."""
+    forces the HBM fallback for the larger boxes, 20352 is the largest box."""
     r32 = small_refs[0]
     pm = small.pmats[::5]
     plain, _, _ = run_gpu(small, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
@@ -136,17 +136,20 @@ def test_lds_staged_matches_unstaged_and_ref32(small, small_refs, chunk, texels)
 
 @pytest.mark.parametrize("storage,bar", [("f16", L2_F16), ("bf16", L2_BF16)])
 def test_lds_staged_half_storage(small, small_refs, storage, bar):
-    """FP16/BF16 (y,z)-quad texels staged in LDS: against the unstaged kernel with the same tile and
-    unroll, and against ref64."""
+    """FP16/BF16 (y,z)-quad texels staged in LDS, with unroll 3 (a third of the shared body through
+    L1) and 4 (a quarter): against the unstaged kernel with the same tile and unroll, and against
+    ref64."""
     r64 = small_refs[2][::5]
     pm = small.pmats[::5]
-    plain, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=4)
-    for chunk, texels in ((64, 0), (16, 1024)):
-        got, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8), unroll=4)
-        check_same_samples(plain, got, f"lds {storage} chunk {chunk} texels {texels}")
-        e = oracle.rel_l2(r64, got)
-        print(f"lds {storage} chunk {chunk} texels {texels}: vs ref64 {e:.3e}")
-        assert e <= bar
+    for unroll in (3, 4):
+        plain, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=-1, tile=(8, 8), unroll=unroll)
+        for chunk, texels in ((64, 0), (16, 1024)):
+            got, _, _ = run_gpu(small, storage, pm=pm, lds_chunk=chunk, lds_texels=texels, tile=(8, 8),
+                                unroll=unroll)
+            check_same_samples(plain, got, f"lds {storage} unroll {unroll} chunk {chunk} texels {texels}")
+            e = oracle.rel_l2(r64, got)
+            print(f"lds {storage} unroll {unroll} chunk {chunk} texels {texels}: vs ref64 {e:.3e}")
+            assert e <= bar
 
 
 def test_lds_staged_real_size_oblique_views():

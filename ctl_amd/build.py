@@ -23,7 +23,7 @@ ARCH = "gfx950"
 
 DEVICE_SOURCES = ["kernels.hip", "processing.hip"]
 HOST_SOURCES = ["api.cpp", "batch.cpp", "geometry.cpp", "phantom.cpp", "serve.cpp", "autotune.cpp",
-                "fileio.cpp", "processing.cpp", "extensions.cpp"]
+                "fileio.cpp", "processing.cpp", "extensions.cpp", "errors.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 # Packed FP32 instructions are kept out of every device code object.

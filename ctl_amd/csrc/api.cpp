@@ -20,34 +20,20 @@
 
 namespace ctlrt {
 
-namespace {
-thread_local std::string g_error;
-} // namespace
-
-ctlproj_status fail(ctlproj_status st, const std::string& msg)
-{
-    g_error = msg;
-    return st;
-}
-
 ctlproj_status hipFail(hipError_t e, const char* what)
 {
-    g_error = std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    fail(CTLPROJ_ERR_DEVICE, std::string(what) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CTLPROJ_ERR_OUT_OF_MEMORY;
     if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return CTLPROJ_ERR_NO_DEVICE;
     return CTLPROJ_ERR_DEVICE;
 }
-
-void clearError() { g_error.clear(); }
-
-const char* lastError() { return g_error.c_str(); }
 
 bool injected(ctlproj_context* ctx)
 {
     int cur = ctx->inject.load();
     while (cur > 0)
         if (ctx->inject.compare_exchange_weak(cur, cur - 1)) {
-            g_error = "injected failure";
+            fail(CTLPROJ_ERR_INJECTED, "injected failure");
             return true;
         }
     return false;

@@ -112,10 +112,13 @@ bool lookup(const std::string& key, ctlk::LaunchShape* shape)
 std::string key(ctlk::Storage st, ctlk::Layout ly, int interpolate, uint32_t channels, uint32_t rows,
                 bool staged)
 {
-    // detector size class: number of 256-pixel steps of the larger side
+    // detector size class: number of 256-pixel steps of the larger side; narrow modules (no wider
+    // than a wavefront tile, whose candidates include the stacked shapes) are a class of their own
     const uint32_t cls = ((channels > rows ? channels : rows) + 255) / 256;
+    const bool narrow = channels <= 16;
     std::ostringstream os;
-    os << "s" << int(st) << "_l" << int(ly) << "_i" << interpolate << "_d" << cls << (staged ? "_lds" : "");
+    os << "s" << int(st) << "_l" << int(ly) << "_i" << interpolate << "_d" << cls << (narrow ? "_n" : "")
+       << (staged ? "_lds" : "");
     return os.str();
 }
 

@@ -106,9 +106,13 @@ ctlproj_status projectAll(ctlproj_projector* p, const ctlproj_volume* const* vol
     return ctlproj_projector_synchronize(p);
 }
 
-ctlproj_status copyOut(const float* d, float* host, size_t n)
+// Copy of a result to the caller's array on the projector's own (non-blocking) stream: a copy on
+// the legacy null stream would wait for every running launch of the process, serving launches
+// included.
+ctlproj_status copyOut(ctlproj_projector* p, const float* d, float* host, size_t n)
 {
-    hipError_t e = hipMemcpy(host, d, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipError_t e = hipMemcpyAsync(host, d, n * sizeof(float), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "result copy");
 }
 
@@ -139,7 +143,7 @@ ctlproj_status ctlproj_project_spectral(ctlproj_projector* p, const ctlproj_volu
     if ((st = ctlproj_spectral_extinction(p->ctx, d_proj.as<float>(), n_materials, p->V, pixels, mu, intensity,
                                           response, n_bins, total_intensity, d_out.as<float>())) != CTLPROJ_OK)
         return st;
-    return copyOut(d_out.as<float>(), out, per);
+    return copyOut(p, d_out.as<float>(), out, per);
 }
 
 ctlproj_status ctlproj_project_linear_dynamic(ctlproj_projector* p, const ctlproj_volume* slope,
@@ -168,5 +172,5 @@ ctlproj_status ctlproj_project_linear_dynamic(ctlproj_projector* p, const ctlpro
     float* a = d_proj.as<float>();
     if ((st = ctlproj_combine_views(p->ctx, a, ca.data(), a + per, cb.data(), p->V, per / p->V, a)) != CTLPROJ_OK)
         return st;
-    return copyOut(a, out, per);
+    return copyOut(p, a, out, per);
 }

@@ -11,7 +11,7 @@
 // dimension, sizes, labels, encoding raw, endian little, optional spacings (+ units mm) and space
 // origin, then the caller's key:=value pairs.
 #include "ctlproj.h"
-#include "runtime.h"
+#include "errors.h"
 
 #include <algorithm>
 #include <cctype>

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "ctlproj.h"
+#include "errors.h"
 #include "kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -55,11 +56,8 @@ struct ctlproj_projector {
 
 namespace ctlrt {
 
-// thread-local error message helpers
-ctlproj_status fail(ctlproj_status st, const std::string& msg);
+// thread-local error message helpers (fail, clearError, lastError: errors.h)
 ctlproj_status hipFail(hipError_t e, const char* what);
-void clearError();
-const char* lastError();
 
 // checks the failure-injection counter of the context
 bool injected(ctlproj_context* ctx);

@@ -8,6 +8,7 @@
 // stream: scan launches queue one behind the other on the device and leave part of it free, and
 // single-view requests (DRRs) take any free slot, so that they run beside a scan launch. A completion thread retires launches, copies finished
 // requests to the caller's host arrays, releases their arena blocks and wakes waiters.
+#include "arena.h"
 #include "ctlproj.h"
 #include "kernels.h"
 #include "runtime.h"
@@ -31,7 +32,6 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-constexpr size_t kAlign = 256;
 // Launches in flight: at most kSlots - 1 of them carry scan chunks, so that a single-view request
 // (DRR) always finds a slot.
 constexpr uint32_t kSlots = 3;
@@ -41,56 +41,9 @@ constexpr uint32_t kSlots = 3;
 constexpr uint32_t kScanFree = 8;
 constexpr uint32_t kSlotItems = 4096;
 
-size_t alignUp(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
-
-// Best-fit allocator over one device allocation; offsets only (host-side bookkeeping).
-class Arena {
-public:
-    void init(size_t size)
-    {
-        size_ = size;
-        free_.clear();
-        free_[0] = size;
-    }
-    bool alloc(size_t bytes, size_t* off)
-    {
-        bytes = alignUp(bytes);
-        auto best = free_.end();
-        for (auto it = free_.begin(); it != free_.end(); ++it)
-            if (it->second >= bytes && (best == free_.end() || it->second < best->second)) best = it;
-        if (best == free_.end()) return false;
-        *off = best->first;
-        const size_t rest = best->second - bytes;
-        const size_t start = best->first;
-        free_.erase(best);
-        if (rest) free_[start + bytes] = rest;
-        used_ += bytes;
-        return true;
-    }
-    void release(size_t off, size_t bytes)
-    {
-        bytes = alignUp(bytes);
-        used_ -= bytes;
-        auto it = free_.emplace(off, bytes).first;
-        auto next = std::next(it);
-        if (next != free_.end() && it->first + it->second == next->first) {
-            it->second += next->second;
-            free_.erase(next);
-        }
-        if (it != free_.begin()) {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == it->first) {
-                prev->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    size_t size() const { return size_; }
-
-private:
-    size_t size_ = 0, used_ = 0;
-    std::map<size_t, size_t> free_;
-};
+using ctlsrv::Arena;
+using ctlsrv::alignUp;
+using ctlsrv::kAlign;
 
 enum class State { Pending, Admitted, Copying, Done, Failed };
 

@@ -3,9 +3,12 @@
 Both extensions call the projection many times in the reference (once per energy bin or material,
 once per view). Here the projections are written to device memory and combined there:
 
-  SpectralEffectsExtension (spectraleffectsextension.cpp:66-156, 232-352): with the linear ray
-    caster, one projection per material density and one kernel that sums the intensities of all
-    energy bins and returns the extinction (ctlproj_spectral_extinction).
+  SpectralEffectsExtension (spectraleffectsextension.cpp:66-156, 232-352): with a linear nested
+    projector (the ray caster), one projection per material density and one kernel that sums the
+    intensities of all energy bins and returns the extinction (ctlproj_spectral_extinction); with a
+    non-linear one (ArealFocalSpotExtension averaging in the intensity domain), one projection per
+    energy bin and material of the bin's attenuation volume (density x mean mass attenuation),
+    summed per bin, as projectNonLinear does (:292-395).
   DynamicProjectorExtension (dynamicprojectorextension.cpp:39-66): for a LinearDynamicVolume
     (lineardynamicvolume.cpp:194-198, value = slope * t + offset) two full-scan projections and a
     per-view combination t_v * P_v(slope) + P_v(offset) (ctlproj_combine_views); any other dynamic
@@ -67,8 +70,9 @@ class SpectralVolume:
 
 
 class SpectralEffectsExtension:
-    """SpectralEffectsExtension with a linear nested projector (the ray caster): per pixel
-    ln(I0 / sum_E i0(E) r(E) exp(-sum_k m_k(E) F(rho_k)))."""
+    """SpectralEffectsExtension: per pixel ln(I0 / sum_E i0(E) r(E) exp(-sum_k m_k(E) F(rho_k)))
+    with a linear nested projector F, ln(I0 / sum_E i0(E) r(E) exp(-sum_k F(m_k(E) rho_k))) with a
+    non-linear one (spectraleffectsextension.cpp:60-98)."""
 
     def __init__(self, projector: RayCasterProjector | None = None,
                  spectral_info: SpectralInformation | None = None, energyBinWidth: float = 10.0):
@@ -118,6 +122,8 @@ class SpectralEffectsExtension:
                                "information is not supported")
         if self._info is None:
             raise RuntimeError("SpectralEffectsExtension: no spectral information set")
+        if not self._nested.isLinear():
+            return self._project_nonlinear(spectral)
         import torch  # device buffers only
         info = self._info
         V, M, R, C_ = self._nested._dims
@@ -146,6 +152,43 @@ class SpectralEffectsExtension:
             _lib.dptr(info.intensities), _lib.fptr(resp) if resp is not None else None, B,
             _lib.dptr(info.total_intensity), C.c_void_p(out.data_ptr())))
         return ProjectionData(out.cpu().numpy().reshape(V, M, R, C_))
+
+
+    def _project_nonlinear(self, spectral) -> ProjectionData:
+        """projectNonLinear / singleBinIntensityNonLinear (spectraleffectsextension.cpp:292-395):
+        for each energy bin with photons, the nested projector projects every material's
+        attenuation volume (density x mean mass attenuation coefficient of the bin, 1/cm -> 1/mm),
+        the projections add up, turn into intensity with the bin's per-view intensities and the
+        detector response; the bins add up and the sum turns into extinction against I0."""
+        info = self._info
+        B, V = info.nbEnergyBins(), info.nbViews()
+        for v in spectral:
+            if len(v.mass_attenuation) != B:
+                raise ValueError("one mass attenuation coefficient per energy bin and material")
+            if not isinstance(v.density, VoxelVolume):
+                raise ValueError("the non-linear spectral path needs host densities (VoxelVolume)")
+        if self._response is not None and len(self._response) != B:
+            raise ValueError("one detector response value per energy bin")
+        total = None
+        for b in range(B):
+            if abs(float(info.intensities[b].sum())) <= 1e-12:  # qFuzzyIsNull: no photons in this bin
+                continue
+            q = None
+            for v in spectral:
+                mu = VoxelVolume(v.density.data * np.float32(v.mass_attenuation[b] * np.float32(0.1)),
+                                 v.density.voxel_size, v.density.offset)
+                pb = self._nested.project(mu).data.astype(np.float64)
+                q = pb if q is None else q + pb
+            if q.shape[0] != V:
+                raise ValueError(f"spectral information holds {V} views, the projector {q.shape[0]}")
+            inten = info.intensities[b].reshape((V,) + (1,) * (q.ndim - 1)) * np.exp(-q)
+            if self._response is not None:
+                inten = inten * float(self._response[b])
+            total = inten if total is None else total + inten
+        if total is None:
+            raise RuntimeError("SpectralEffectsExtension: no energy bin holds photons")
+        i0 = info.total_intensity.reshape((V,) + (1,) * (total.ndim - 1))
+        return ProjectionData(np.log(i0 / total).astype(np.float32))
 
 
 class LinearDynamicVolume:

@@ -155,6 +155,58 @@ def test_spectral_effects_against_oracle():
     assert not ext.isLinear()
 
 
+def test_spectral_effects_nonlinear_nested_projector():
+    """The non-linear path (spectraleffectsextension.cpp:292-395) with ArealFocalSpotExtension
+    averaging in the intensity domain as the nested projector: per bin and material the areal
+    projection of the attenuation volume, against the oracle (ref32 at each focal-spot position,
+    averaged in the intensity domain in FP64, summed over materials, combined over bins in FP64).
+    With the linear ray caster forced onto the same path the result equals the linear path's to
+    FP32 rounding, since F(m rho) = m F(rho)."""
+    w = W.small(views=4)
+    spot = (2.0, 1.0)
+    setup = small_setup(spot=spot)
+    d1 = np.ascontiguousarray(w.volume / w.volume.max() * 1.2, np.float32)
+    d2 = np.ascontiguousarray(w.volume[::-1, :, ::-1] / w.volume.max() * 0.8, np.float32)
+    energies = [30.0, 60.0, 90.0]
+    rng = np.random.default_rng(23)
+    inten = rng.uniform(0.5, 2.0, (3, 4))
+    ma1, ma2 = [0.375, 0.206, 0.169], [1.33, 0.315, 0.186]
+    resp = [0.8, 1.0, 0.9]
+    vols = [SpectralVolume(VoxelVolume(d1, (1.0, 1.0, 1.0)), ma1), SpectralVolume(VoxelVolume(d2, (1.0, 1.0, 1.0)), ma2)]
+    nested = RayCasterProjector()
+    nested.settings().raySampling = 0.1
+    areal = ArealFocalSpotExtension(nested, (2, 2), False)
+    assert not areal.isLinear()
+    ext = SpectralEffectsExtension(areal, SpectralInformation(energies, inten))
+    ext.setDetectorResponse(resp)
+    ext.configure(setup)
+    got = ext.projectComposite(vols).data
+    grid = areal.discretizationGrid()
+    pms = [GeometryEncoder.encodeFullGeometry(setup, (x * spot[0], y * spot[1], 0.0)) for x, y in grid]
+    total = 0.0
+    for b in range(3):
+        q = 0.0
+        for d, ma in ((d1, ma1), (d2, ma2)):
+            mu = np.ascontiguousarray(d * np.float32(ma[b] * np.float32(0.1)), np.float32)
+            subs = np.stack([oracle.project(mu, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), pm, 256, 128, 0.1)[0]
+                             .astype(np.float64) for pm in pms])
+            q = q + -np.log(np.exp(-subs).mean(0))
+        total = total + inten[b].reshape(4, 1, 1, 1) * np.exp(-q) * resp[b]
+    ref = np.log(inten.sum(0).reshape(4, 1, 1, 1) / total)
+    e = oracle.rel_l2(ref, got)
+    print(f"spectral effects, non-linear nested projector: rel L2 vs oracle {e:.3e}")
+    assert got.shape == (4, 1, 128, 256) and e <= 1e-4
+    # the linear nested projector forced onto the non-linear path agrees with the linear path
+    lin = SpectralEffectsExtension(nested, SpectralInformation(energies, inten))
+    lin.setDetectorResponse(resp)
+    lin.configure(setup)
+    a = lin.projectComposite(vols).data
+    b_ = lin._project_nonlinear(vols).data
+    d = oracle.rel_l2(a, b_)
+    print(f"linear vs non-linear path with the linear ray caster: rel L2 {d:.3e}")
+    assert d <= 1e-5
+
+
 def test_dynamic_linear_volume_against_oracle():
     """DynamicProjectorExtension on a LinearDynamicVolume: t_v * P_v(slope) + P_v(offset) on the
     device against ref32 projections of slope * t_v + offset, one view at a time."""

@@ -97,17 +97,26 @@ ctlk::LaunchShape shapeFor(const ctlproj_settings& s, ctlk::Storage st, ctlk::La
 }
 
 namespace {
-// Limits of the gather addressing: 24-bit row/slice multiplies and 32-bit buffer offsets.
+// Padded arrays below this size are one buffer of 32-bit byte offsets; larger ones are read with
+// 64-bit addresses (VolumeView::large).
+constexpr uint64_t kLargeBytes = (1ull << 32) - 16;
+
+// Limits of the gather addressing: 24-bit multiplies of texel rows and slices, 32-bit byte
+// offsets below 4 GiB, and above it 32-bit texel row indices in the interleaved layout.
 ctlproj_status checkPaddedSize(const uint32_t n[3], uint32_t f, ctlk::Storage st, ctlk::Layout ly,
                                size_t* bytes)
 {
     const uint64_t px = uint64_t(n[0]) * f + 2, py = uint64_t(n[1]) * f + 2, pz = uint64_t(n[2]) * f + 2;
     const uint64_t eb = ctlk::texelBytes(st, ly);
-    if (px * py * eb >= (1ull << 24))
-        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume slice too large (X*Y*bytes must be < 16 MiB)");
+    if (px * py >= (1ull << 24) || pz >= (1ull << 24))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume slice too large ((X+2)*(Y+2) must be < 2^24 voxels)");
     const uint64_t b = px * py * pz * eb;
-    if (b >= (1ull << 32) - 16)
-        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume too large for one gather buffer (< 4 GiB)");
+    if (b >= kLargeBytes) {
+        if (ly != ctlk::Layout::Interleaved)
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volumes of 4 GiB or more need the interleaved layout");
+        if (py * pz >= (1ull << 32))
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume has too many texel rows ((Y+2)*(Z+2) must be < 2^32)");
+    }
     *bytes = size_t(b);
     return CTLPROJ_OK;
 }
@@ -119,12 +128,22 @@ ctlk::VolumeView makeView(void* d, const uint32_t n[3], ctlk::Storage st, ctlk::
     for (int k = 0; k < 3; ++k) v.n[k] = n[k];
     v.pitch_y = n[0] + 2;
     v.pitch_z = (n[0] + 2) * (n[1] + 2);
-    v.bytes = uint32_t(bytes);
     v.storage = st;
     v.layout = ly;
+    v.row_bytes = uint32_t((n[0] + 2) * ctlk::texelBytes(st, ly));
+    v.rows = (n[1] + 2) * (n[2] + 2);
+    // CTLPROJ_VOLUME_WIDE=1 reads every interleaved volume as volumes of 4 GiB or more are read
+    // (the same kernels, at sizes that small tests reach)
+    const char* wide = std::getenv("CTLPROJ_VOLUME_WIDE");
+    const bool force = wide && std::strcmp(wide, "1") == 0 && ly == ctlk::Layout::Interleaved;
+    v.large = (bytes >= kLargeBytes || force) ? 1u : 0u;
+    // a large volume's buffer covers its first 4 GiB: the probes' streams and nothing else
+    v.bytes = uint32_t(std::min<uint64_t>(bytes, kLargeBytes));
     return v;
 }
 } // namespace
+
+bool largeVolume(const ctlproj_volume* vol) { return vol && vol->view.large; }
 
 ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings& s,
                                hipStream_t stream, EffectiveVolume& ev)
@@ -968,10 +987,10 @@ ctlproj_status ctlproj_diag_gather_probe(ctlproj_projector* p, const ctlproj_vol
         if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
         if (p->V == 0) return fail(CTLPROJ_ERR_NOT_CONFIGURED, "no views to probe");
         if (!p->s.interpolate || p->s.rays_per_pixel[0] != 1 || p->s.rays_per_pixel[1] != 1 ||
-            p->s.volume_upsampling != 1 || vol->view.layout != ctlk::Layout::Interleaved)
+            p->s.volume_upsampling != 1 || vol->view.layout != ctlk::Layout::Interleaved || vol->view.large)
             return fail(CTLPROJ_ERR_INVALID_ARGUMENT,
                         "the gather probe needs trilinear, one ray per pixel, no upsampling and an "
-                        "interleaved volume");
+                        "interleaved volume below 4 GiB");
         EffectiveVolume ev;
         if ((st = effectiveVolume(vol, p->s, p->stream, ev)) != CTLPROJ_OK) return st;
         if ((st = prepareGeometry(p, ev)) != CTLPROJ_OK) return st;

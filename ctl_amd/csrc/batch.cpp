@@ -191,6 +191,7 @@ ctlproj_status ctlproj_batch_add(ctlproj_batch* b, const ctlproj_volume* vol, co
     if (b->finalized) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "batch is finalized");
     if (!vol || !vol->d_data) return fail(CTLPROJ_ERR_NO_DATA, "no or contradictory data in volume object");
     if (vol->ctx != b->ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume belongs to another context");
+    if (largeVolume(vol)) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "batches hold volumes below 4 GiB only");
     if (views == 0 || modules == 0 || channels == 0 || rows == 0 || !pmats)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "every batch entry needs views and a detector");
     if (out_count != size_t(views) * modules * rows * channels || !device_out)
@@ -238,6 +239,8 @@ ctlproj_status ctlproj_batch_finalize(ctlproj_batch* b, int autotune)
     for (auto& in : b->insts) {
         ctlproj_status st = effectiveVolume(in->vol, in->s, b->stream, in->ev);
         if (st != CTLPROJ_OK) return st;
+        if (in->ev.view.large) // upsampled to 4 GiB or more
+            return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "batches hold volumes below 4 GiB only");
         const size_t n = in->pmats.size();
         if ((e = hipMalloc(&in->d_pmats, n * sizeof(double))) != hipSuccess ||
             (e = hipMalloc(&in->d_geom, size_t(in->V) * in->M * sizeof(ctlk::RayGeom))) != hipSuccess ||

@@ -25,14 +25,21 @@ inline size_t texelBytes(Storage s, Layout l) { return l == Layout::Interleaved 
 
 // A resident volume in HBM: (X+2) x (Y+2) x (Z+2) texels, x fastest, with a one-texel zero
 // border on every side, so that every tap of a sample inside the support is an in-bounds read.
+// A volume of 4 GiB or more (`large`, interleaved layout only) is beyond the 32-bit offsets of a
+// buffer: its gathers use 64-bit addresses (texel row y + z*(Y+2) times row_bytes, plus x), and
+// its LDS staging loads a buffer window that starts at the staged box's first slice.
 struct VolumeView {
     const void* data;
     uint32_t n[3];      // interior voxel counts
     uint32_t pitch_y;   // X + 2        (texels per row)
     uint32_t pitch_z;   // (X+2)*(Y+2)  (texels per slice)
-    uint32_t bytes;     // total bytes of the padded array (buffer range for the gather loads)
+    uint32_t bytes;     // total bytes of the padded array (buffer range for the gather loads); for
+                        // a large volume the largest 32-bit range, used by no gather
     Storage storage;
     Layout layout;
+    uint32_t large;     // 1: 64-bit addressing
+    uint32_t row_bytes; // bytes per texel row, (X+2) * texel bytes
+    uint32_t rows;      // texel rows, (Y+2) * (Z+2)
 };
 
 // Volume geometry after upsampling, as the host passes it (FP32, raycasterprojector.cpp:169-179).
@@ -133,6 +140,7 @@ inline bool stageable(Storage, Layout ly, const RayParams& rp)
 struct Pitch {
     uint32_t y, z, yz; // byte offsets of the (y+1), (z+1), (y+1,z+1) texel rows
     uint32_t ty, tz;   // texels per row / slice
+    uint32_t py;       // texel rows per slice (64-bit addressing of large volumes)
 };
 
 // Launch constants of one projection (tile decomposition, step, box, texel pitches).

@@ -279,14 +279,19 @@ __device__ __forceinline__ int waveMax(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-template <Storage ST, Layout LY, bool INTERP>
+// LARGE: a volume of 4 GiB or more, read at 64-bit addresses from `base` (interleaved layout
+// only); otherwise through the buffer `rs`.
+template <Storage ST, Layout LY, bool INTERP, bool LARGE = false>
 __device__ __forceinline__ float sampleAt(__amdgpu_buffer_rsrc_t rs, const Pitch& p, float fi,
-                                          const float s[3], const float c[3])
+                                          const float s[3], const float c[3], const void* base = nullptr)
 {
     const float qx = fmaf(fi, s[0], c[0]);
     const float qy = fmaf(fi, s[1], c[1]);
     const float qz = fmaf(fi, s[2], c[2]);
-    return INTERP ? trilinear<ST, LY>(rs, p, qx, qy, qz) : nearest<ST, LY>(rs, p, qx, qy, qz);
+    if constexpr (LARGE)
+        return INTERP ? trilinearWide<ST>(base, p, qx, qy, qz) : nearestWide<ST>(base, p, qx, qy, qz);
+    else
+        return INTERP ? trilinear<ST, LY>(rs, p, qx, qy, qz) : nearest<ST, LY>(rs, p, qx, qy, qz);
 }
 
 // Processes logical tile b of one projection: one thread per detector pixel. All rays of a
@@ -309,7 +314,7 @@ __device__ __forceinline__ uint32_t gatherBits(uint32_t lane, uint32_t e)
            ((lane >> ((e >> 18) & 7u)) & ((e >> 21) & 63u));
 }
 
-template <Storage ST, Layout LY, bool INTERP, int UNROLL>
+template <Storage ST, Layout LY, bool INTERP, int UNROLL, bool LARGE = false>
 __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                          const RayParams& rp, const CastConst& kc,
                                          float* __restrict__ out, uint32_t b, uint32_t wave)
@@ -325,7 +330,7 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
     const bool pixel = x < rp.channels && y < rp.rows;
 
     const RayGeom g = geom[vm];
-    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes); // unused by LARGE gathers
 
     // kernel prologue of raycasterprojector_interp.cl:39-40
     const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
@@ -352,7 +357,7 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                 // head [lo, a) and, when there is no common range, everything
                 for (int i = lo; i < a; ++i)
                     if (act && uint32_t(i - i0) <= span)
-                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, float(i), ray.s, g.c);
+                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
                 // body [a, bb]: every active ray samples
                 if (act) {
                     int i = a;
@@ -360,15 +365,15 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                     for (; i + UNROLL - 1 <= bb; i += UNROLL, fi += float(UNROLL)) {
 #pragma unroll
                         for (int u = 0; u < UNROLL; ++u)
-                            part[u] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, fi + float(u), ray.s, g.c);
+                            part[u] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi + float(u), ray.s, g.c, vol.data);
                     }
                     for (; i <= bb; ++i, fi += 1.0f)
-                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, fi, ray.s, g.c);
+                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
                 }
                 // tail (bb, hi]
                 for (int i = bb + 1; i <= hi; ++i)
                     if (act && uint32_t(i - i0) <= span)
-                        part[0] += sampleAt<ST, LY, INTERP>(rs, kc.pitch, float(i), ray.s, g.c);
+                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
                 float sum = 0.0f;
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) sum += part[u];
@@ -382,13 +387,13 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
     }
 }
 
-template <Storage ST, Layout LY, bool INTERP, int UNROLL>
+template <Storage ST, Layout LY, bool INTERP, int UNROLL, bool LARGE = false>
 __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
                                                          RayParams rp, CastConst kc,
                                                          float* __restrict__ out)
 {
-    castTile<ST, LY, INTERP, UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x),
-                                     threadIdx.x >> 6);
+    castTile<ST, LY, INTERP, UNROLL, LARGE>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x),
+                                            threadIdx.x >> 6);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -489,7 +494,7 @@ __device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, in
         if (live && uint32_t(i - a) <= span) part[0] += f(float(i));
 }
 
-template <Storage ST, int UNROLL>
+template <Storage ST, int UNROLL, bool LARGE = false>
 __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                                const RayParams& rp, const CastConst& kc,
                                                float* __restrict__ out, uint32_t b, uint32_t wave,
@@ -506,7 +511,7 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
     const bool pixel = x < rp.channels && y < rp.rows;
 
     const RayGeom g = geom[vm];
-    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes);
+    const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes); // unused by LARGE gathers
     float acc = 0.0f;
     if (g.valid) { // uniform over the workgroup: one (view, module) per workgroup
         // one ray per pixel: the sub-ray coordinate x - 1/2 + 1/2 is x exactly
@@ -578,8 +583,10 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             // with a box to stage (16-bit coordinates valid), 0xffff marks a chunk in which no lane
             // of the workgroup samples; without, sampling below finds no live lane
             if (kc.lds_texels && o[0] == 0xffff) continue; // uniform
+            // a large volume stages through a window of the box's slices, addressed with 32 bits
             const bool small = kc.lds_texels && nb[0] >= 1 && nb[0] <= 256 && nb[1] >= 1 && nb[1] <= 256 &&
-                               nb[2] >= 1 && nb[2] <= 256;
+                               nb[2] >= 1 && nb[2] <= 256 &&
+                               (!LARGE || uint64_t(nb[2] + 1) * kc.pitch.tz * 8u < 0xffff0000ull);
             // LDS pitches: even rows (a 16-byte load never straddles two rows), not a multiple of
             // 16 texels (rows would share LDS banks)
             int px = (nb[0] + 1) & ~1;
@@ -606,11 +613,25 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     dy = drow - dz * nb[1];
                     if (dy < 0) { --dz; dy += nb[1]; } else if (dy >= nb[1]) { ++dz; dy -= nb[1]; }
                 }
+                // the staging loads' buffer: the volume, or for a large volume the window that starts at
+                // the box's first slice (its slices then count from 0)
+                __amdgpu_buffer_rsrc_t win = rs;
+                uint32_t z0 = uint32_t(o[2]);
+                if constexpr (LARGE) {
+                    const uint64_t slice = uint64_t(kc.pitch.tz) * 8u;
+                    const uint64_t total = uint64_t(vol.rows) * vol.row_bytes;
+                    // o is the workgroup's box origin: uniform, so the window is a scalar resource
+                    const uint64_t first = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(o[2]))) * slice;
+                    const uint64_t rest = total > first ? total - first : 0;
+                    win = makeRsrc(static_cast<const char*>(vol.data) + first,
+                                   uint32_t(rest < 0xffff0000ull ? rest : 0xffff0000ull));
+                    z0 = 0;
+                }
                 for (int k = int(wave); k < n_ins; k += 4) {
                     // texels past the box land in padding of the LDS box; past the volume they read 0
                     const uint32_t src = (uint32_t(o[0] + cx) + __umul24(uint32_t(o[1] + cy), kc.pitch.ty) +
-                                          uint32_t(o[2] + cz) * kc.pitch.tz) << 3;
-                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(box + k * 128), 16, src, 0, 0, 0);
+                                          (z0 + uint32_t(cz)) * kc.pitch.tz) << 3;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(win, (lds_void*)(box + k * 128), 16, src, 0, 0, 0);
                     cx += dx;
                     cy += dy;
                     cz += dz;
@@ -637,11 +658,11 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     // vector L1 instead (the box was just read from there): the LDS pipe and the
                     // texture path then share the taps. 256^3 / 1024 x 768, 90 views: a third
                     // 19.0 ms, a quarter 19.5, half 21.1, none 22.7
-                    return sampleAt<ST, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
+                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
                 });
             } else {
                 auto gather = [&](float fi) {
-                    return sampleAt<ST, Layout::Interleaved, true>(rs, kc.pitch, fi, ray.s, g.c);
+                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
                 };
                 sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, gather, gather);
             }
@@ -663,15 +684,15 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
     }
 }
 
-template <Storage ST, int UNROLL>
+template <Storage ST, int UNROLL, bool LARGE = false>
 __global__ __launch_bounds__(kBlock) void staged_kernel(VolumeView vol, const RayGeom* __restrict__ geom,
                                                         RayParams rp, CastConst kc, float* __restrict__ out)
 {
     extern __shared__ __attribute__((aligned(16))) float2 s_box[];
     __shared__ int s_red[2][4][8];
     static_assert(sizeof(s_red) <= kStaticLds, "static LDS beyond the reserve");
-    castTileStaged<ST, UNROLL>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
-                           s_box, s_red);
+    castTileStaged<ST, UNROLL, LARGE>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
+                                      s_box, s_red);
 }
 
 // Persistent batch kernel: a fixed grid of workgroups drains a queue of tiles that spans many
@@ -1200,6 +1221,7 @@ CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayPa
         const uint32_t eb = uint32_t(texelBytes(vol->storage, vol->layout));
         kc.pitch.ty = vol->pitch_y;
         kc.pitch.tz = vol->pitch_z;
+        kc.pitch.py = vol->n[1] + 2;
         kc.pitch.y = vol->pitch_y * eb;
         kc.pitch.z = vol->pitch_z * eb;
         kc.pitch.yz = (vol->pitch_y + vol->pitch_z) * eb;
@@ -1214,6 +1236,19 @@ hipError_t launchCastU(const VolumeView& vol, const RayGeom* g, const RayParams&
                        const CastConst& kc, uint32_t unroll, uint32_t blocks, uint32_t lds, float* out,
                        hipStream_t s)
 {
+    if constexpr (LY == Layout::Interleaved) {
+        if (vol.large) {
+            switch (unroll) {
+            case 1: raycast_kernel<ST, LY, INTERP, 1, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+            case 3:
+            case 4: raycast_kernel<ST, LY, INTERP, 4, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+            default: raycast_kernel<ST, LY, INTERP, 2, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+            }
+            return hipGetLastError();
+        }
+    } else if (vol.large) {
+        return hipErrorInvalidValue; // large volumes are interleaved
+    }
     switch (unroll) {
     case 1: raycast_kernel<ST, LY, INTERP, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     case 3: // the unstaged kernels run 3 as 4
@@ -1227,6 +1262,15 @@ template <Storage ST>
 hipError_t launchStaged(const VolumeView& vol, const RayGeom* g, const RayParams& rp, const CastConst& kc,
                         uint32_t unroll, uint32_t blocks, uint32_t lds, float* out, hipStream_t s)
 {
+    if (vol.large) {
+        switch (unroll) {
+        case 1: staged_kernel<ST, 1, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+        case 3: staged_kernel<ST, 3, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+        case 4: staged_kernel<ST, 4, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+        default: staged_kernel<ST, 2, true><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
+        }
+        return hipGetLastError();
+    }
     switch (unroll) {
     case 1: staged_kernel<ST, 1><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;
     case 3: staged_kernel<ST, 3><<<blocks, kBlock, lds, s>>>(vol, g, rp, kc, out); break;

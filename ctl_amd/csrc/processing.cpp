@@ -30,6 +30,8 @@ ctlproj_status begin(const ctlproj_volume* vol)
     hipError_t e = hipSetDevice(vol->ctx->device);
     if (e != hipSuccess) return hipFail(e, "hipSetDevice");
     if (injected(vol->ctx)) return CTLPROJ_ERR_INJECTED;
+    if (largeVolume(vol))
+        return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume processing reads volumes below 4 GiB only");
     return CTLPROJ_OK;
 }
 

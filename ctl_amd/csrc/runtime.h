@@ -73,6 +73,8 @@ struct EffectiveVolume {
 
 ctlproj_status effectiveVolume(const ctlproj_volume* vol, const ctlproj_settings& s,
                                hipStream_t stream, EffectiveVolume& ev);
+// True for a volume of 4 GiB or more: only the projections (ctlproj_project*) read those.
+bool largeVolume(const ctlproj_volume* vol);
 
 // Device allocation released on scope exit (callers synchronise their stream before that).
 struct DeviceBuffer {

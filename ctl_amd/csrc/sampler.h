@@ -168,6 +168,51 @@ __device__ __forceinline__ float nearest(__amdgpu_buffer_rsrc_t r, const Pitch& 
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Volumes of 4 GiB or more (VolumeView::large): a buffer's offsets (index * stride included) are
+// 32-bit, so these read with 64-bit addresses, row (y + z*(Y+2)) times the row bytes plus the
+// byte offset of x. Every tap of a sample in the support (blo <= q < n+1) lies inside the array.
+// Only the interleaved layouts are read this way.
+
+typedef uint32_t wideu4 __attribute__((ext_vector_type(4), aligned(8))); // 8-byte aligned texel pairs
+typedef uint32_t wideu2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ const char* wideAddr(const void* base, const Pitch& p, uint32_t ix, uint32_t iy,
+                                                uint32_t iz)
+{
+    const uint32_t row = iy + __umul24(iz, p.py);
+    return static_cast<const char*>(base) + (uint64_t(row) * p.y + (ix << 3));
+}
+
+template <Storage ST>
+__device__ __forceinline__ float trilinearWide(const void* base, const Pitch& p, float qx, float qy, float qz)
+{
+    const uint32_t ix = uint32_t(int(qx)), iy = uint32_t(int(qy)), iz = uint32_t(int(qz)); // q >= 0
+    const float wx = __builtin_amdgcn_fractf(qx);
+    const float wy = __builtin_amdgcn_fractf(qy);
+    const float wz = __builtin_amdgcn_fractf(qz);
+    const char* t = wideAddr(base, p, ix, iy, iz);
+    if constexpr (ST == Storage::F32) { // (v, dz) texels at x and x+1 of rows y and y+1
+        const float4 a = __builtin_bit_cast(float4, *reinterpret_cast<const wideu4*>(t));
+        const float4 b = __builtin_bit_cast(float4, *reinterpret_cast<const wideu4*>(t + p.y));
+        return triDeltaZ(make_float2(a.x, a.y), make_float2(a.z, a.w), make_float2(b.x, b.y),
+                         make_float2(b.z, b.w), wx, wy, wz);
+    } else { // (y,z)-quads at x and x+1
+        const wideu4 q = *reinterpret_cast<const wideu4*>(t);
+        return quadTri<ST>(q.x, q.y, q.z, q.w, wx, wy, wz);
+    }
+}
+
+template <Storage ST>
+__device__ __forceinline__ float nearestWide(const void* base, const Pitch& p, float qx, float qy, float qz)
+{
+    const wideu2 t = *reinterpret_cast<const wideu2*>(
+        wideAddr(base, p, uint32_t(int(qx)), uint32_t(int(qy)), uint32_t(int(qz)))); // the texel at x
+    if constexpr (ST == Storage::F32) return __uint_as_float(t.x);
+    else if constexpr (ST == Storage::F16) return float(__builtin_bit_cast(_Float16, uint16_t(t.x & 0xffffu)));
+    else return __uint_as_float(t.x << 16);
+}
+
 // OpenCL linear image read at unnormalised coordinate u (the reference's read_imagef with
 // CLK_FILTER_LINEAR | CLK_ADDRESS_CLAMP) = sampleTexel at q = u + 1/2: zero wherever all eight
 // taps are border texels, that is unless 0 <= q < n + 1 on every axis (nhi = n + 1). NaN

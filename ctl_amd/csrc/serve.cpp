@@ -511,6 +511,7 @@ ctlproj_status ctlproj_server_submit(ctlproj_server* S, const ctlproj_volume* vo
     if (!S) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "server is NULL");
     if (!vol || !vol->d_data) return fail(CTLPROJ_ERR_NO_DATA, "no or contradictory data in volume object");
     if (vol->ctx != S->ctx) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "volume belongs to another context");
+    if (largeVolume(vol)) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "the server projects volumes below 4 GiB only");
     if (views == 0 || modules == 0 || channels == 0 || rows == 0 || !pmats || !host_out)
         return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "a request needs views, a detector and an output");
     if (out_count != size_t(views) * modules * rows * channels)

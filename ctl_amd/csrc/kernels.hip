@@ -329,16 +329,23 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
     const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
 
-    const RayGeom g = geom[vm];
+    const RayGeom* gp = geom + vm;
     const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes); // unused by LARGE gathers
 
     // kernel prologue of raycasterprojector_interp.cl:39-40
     const float ipx = 1.0f / float(rp.rpp_x), ipy = 1.0f / float(rp.rpp_y);
 
     float acc = 0.0f;
-    if (g.valid) {
+    if (gp->valid) {
         for (uint32_t ra = 0; ra < rp.rpp_x; ++ra)
             for (uint32_t rb = 0; rb < rp.rpp_y; ++rb) {
+                // the 160-byte record is read again for each sub-ray (the empty asm hides the
+                // pointer's value): held in registers across the sample loop, its FP64 QR pushed the
+                // persistent kernels to 124 VGPRs, where the compiler issued the unrolled gathers one
+                // at a time (served FP16 scans 1.6x slower)
+                const RayGeom* gr = gp;
+                asm volatile("" : "+s"(gr));
+                const RayGeom& g = *gr;
                 Ray ray = setupRay(g, subRayCoord(x, ipx, ra), subRayCoord(y, ipy, rb), kc.inc, kc.vs,
                                    kc.nhi, kc.blo);
                 if (!pixel) ray.hit = false;

@@ -130,6 +130,7 @@ SIGNATURES: dict[str, tuple] = {
     "ctlproj_projector_synchronize": (_i, [_vp]),
     "ctlproj_projector_last_kernel_ms": (_i, [_vp, _f32p]),
     "ctlproj_projector_count_samples": (_i, [_vp, _vp, _u64p]),
+    "ctlproj_projector_count_taken": (_i, [_vp, _vp, _vp, _sz, _u64p]),
     "ctlproj_encode_trajectory": (_i, [C.POINTER(Gantry), C.POINTER(Detector), C.POINTER(Trajectory),
                                        _u32, _f64p, _sz]),
     "ctlproj_encode_carm_poses": (_i, [C.c_double, C.POINTER(Detector), _f64p, _f64p, _u32, _f64p, _sz]),

@@ -234,6 +234,21 @@ ctlproj_status createVolume(ctlproj_context* ctx, const float* d_src, const uint
         v->offset_mm[k] = offset_mm[k];
     }
     v->view = makeView(v->d_data, dims, st, ly, bytes);
+    // The occupancy table lets the ray casters skip chunks of samples that read only zero voxels
+    // (their sums are unchanged). CTLPROJ_EMPTY_SKIP=0 leaves it out: every sample is then taken.
+    const char* skip = std::getenv("CTLPROJ_EMPTY_SKIP");
+    if (!(skip && std::strcmp(skip, "0") == 0)) {
+        if ((e = hipMalloc(reinterpret_cast<void**>(&v->d_occ), ctlk::occBytes(dims))) != hipSuccess ||
+            (e = ctlk::launchOccupancy(v->view, v->d_occ, ctx->stream)) != hipSuccess ||
+            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+            if (v->d_occ) (void)hipFree(v->d_occ);
+            (void)hipFree(v->d_data);
+            delete v;
+            return hipFail(e, "volume occupancy table");
+        }
+        v->view.occ = v->d_occ;
+        for (int k = 0; k < 3; ++k) v->view.occ_n[k] = ctlk::occBricks(dims[k]);
+    }
     *out = v;
     return CTLPROJ_OK;
 }
@@ -316,12 +331,13 @@ ctlproj_status checkProject(ctlproj_projector* p, const ctlproj_volume* vol, con
 }
 
 ctlproj_status launchCast(ctlproj_projector* p, const EffectiveVolume& ev, uint32_t v0, uint32_t nv,
-                          float* d_out, int accumulate = 0, const ctlk::RayGeom* geom = nullptr)
+                          float* d_out, int accumulate = 0, const ctlk::RayGeom* geom = nullptr,
+                          unsigned long long* tally = nullptr)
 {
     const ctlk::RayParams rp = rayParams(p, nv);
     const ctlk::LaunchShape sh = shapeFor(p->s, ev.view.storage, ev.view.layout, p->C, p->R);
     hipError_t e = ctlk::launchRaycast(ev.view, ev.geom, (geom ? geom : p->d_geom) + size_t(v0) * p->M, rp, sh,
-                                       d_out, p->stream, accumulate);
+                                       d_out, p->stream, accumulate, tally);
     return e == hipSuccess ? CTLPROJ_OK : hipFail(e, "raycast kernel");
 }
 
@@ -485,6 +501,7 @@ ctlproj_status ctlproj_volume_destroy(ctlproj_volume* vol)
     if (!vol) return CTLPROJ_OK;
     (void)hipSetDevice(vol->ctx->device);
     if (vol->d_data) (void)hipFree(vol->d_data);
+    if (vol->d_occ) (void)hipFree(vol->d_occ);
     if (vol->d_brick) (void)hipFree(vol->d_brick);
     delete vol;
     return CTLPROJ_OK;
@@ -856,6 +873,29 @@ ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlpr
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) return hipFail(e, "count kernel");
     *n_samples = h;
+    return CTLPROJ_OK;
+}
+
+ctlproj_status ctlproj_projector_count_taken(ctlproj_projector* p, const ctlproj_volume* vol, float* device_out,
+                                            size_t out_count, uint64_t* n_taken)
+{
+    if (!n_taken) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "n_taken is NULL");
+    ctlproj_status st = checkProject(p, vol, device_out, out_count);
+    if (st != CTLPROJ_OK) return st;
+    *n_taken = 0;
+    if (p->V == 0) return CTLPROJ_OK;
+    if ((st = setDevice(p->ctx)) != CTLPROJ_OK) return st;
+    EffectiveVolume ev;
+    if ((st = effectiveVolume(vol, p->s, p->stream, ev)) != CTLPROJ_OK) return st;
+    if ((st = prepareGeometry(p, ev)) != CTLPROJ_OK) return st;
+    hipError_t e = hipMemsetAsync(p->d_count, 0, sizeof(unsigned long long), p->stream);
+    if (e != hipSuccess) return hipFail(e, "hipMemsetAsync");
+    if ((st = launchCast(p, ev, 0, p->V, device_out, 0, nullptr, p->d_count)) != CTLPROJ_OK) return st;
+    unsigned long long h = 0;
+    e = hipMemcpyAsync(&h, p->d_count, sizeof h, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return hipFail(e, "sample tally");
+    *n_taken = h;
     return CTLPROJ_OK;
 }
 

@@ -40,7 +40,22 @@ struct VolumeView {
     uint32_t large;     // 1: 64-bit addressing
     uint32_t row_bytes; // bytes per texel row, (X+2) * texel bytes
     uint32_t rows;      // texel rows, (Y+2) * (Z+2)
+    // Occupancy of the padded voxel grid in bricks of kOccBrick^3 voxels, as a summed-area table:
+    // occ[((k * (oy + 1)) + j) * (ox + 1) + i] = number of bricks holding a non-zero voxel (NaN
+    // counts) among bricks [0, i) x [0, j) x [0, k), for (ox, oy, oz) = occ_n. NULL: no table.
+    const uint32_t* occ;
+    uint32_t occ_n[3];
 };
+
+// Edge of the occupancy table's bricks, in voxels of the padded grid.
+constexpr uint32_t kOccShift = 2;
+constexpr uint32_t kOccBrick = 1u << kOccShift;
+// Bricks per axis of a padded grid of n + 2 voxels, and the table's size in bytes.
+inline uint32_t occBricks(uint32_t n) { return (n + 2 + kOccBrick - 1) >> kOccShift; }
+inline size_t occBytes(const uint32_t n[3])
+{
+    return size_t(occBricks(n[0]) + 1) * (occBricks(n[1]) + 1) * (occBricks(n[2]) + 1) * sizeof(uint32_t);
+}
 
 // Volume geometry after upsampling, as the host passes it (FP32, raycasterprojector.cpp:169-179).
 struct VolumeGeom {
@@ -156,6 +171,9 @@ struct CastConst {
     uint32_t lds_texels; // LDS-staged kernel: capacity of the LDS box in 8-byte texels
     uint32_t accumulate; // 1: add to the output instead of overwriting it
     uint32_t lane_x, lane_y; // lane -> pixel in a wavefront tile: bit-gather codes (makeCastConst)
+    // diagnostics: when not NULL, the casters add the number of samples they take (those of chunks
+    // and segments skipped as empty excluded) here
+    unsigned long long* tally;
 };
 
 CastConst makeCastConst(const VolumeView* vol, const VolumeGeom& vg, const RayParams& rp,
@@ -198,7 +216,8 @@ hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, 
 // adds the projection to what d_out holds.
 hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                          const RayParams& rp, const LaunchShape& shape, float* d_out,
-                         hipStream_t s, int accumulate = 0);
+                         hipStream_t s, int accumulate = 0,
+                         unsigned long long* tally = nullptr);
 
 // Reference loop-count of all rays, added into *d_count.
 hipError_t launchCount(const VolumeGeom& vg, const RayGeom* d_geom, const RayParams& rp,
@@ -209,6 +228,9 @@ hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Stora
                       void* d_dst, hipStream_t s);
 // padded storage -> padded storage upsampled by f (f >= 1), same storage and layout.
 hipError_t launchUpsample(const VolumeView& src, uint32_t f, void* d_dst, hipStream_t s);
+// Builds view.occ's table (occBytes(view.n) bytes at d_occ, which view.occ need not point to yet)
+// from the view's voxels.
+hipError_t launchOccupancy(const VolumeView& view, uint32_t* d_occ, hipStream_t s);
 // Mean of the interior of a padded volume, into *d_sum (double, accumulated).
 hipError_t launchSum(const VolumeView& src, double* d_sum, hipStream_t s);
 // Fill a one-texel padded volume (1x1x1 interior) with the value *d_sum / count.

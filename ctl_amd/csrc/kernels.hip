@@ -279,6 +279,73 @@ __device__ __forceinline__ int waveMax(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+__device__ __forceinline__ float waveMinF(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+
+__device__ __forceinline__ float waveMaxF(float v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+
+// Adds this wave's lanes' counts to *tally (lane 0's atomic); all lanes active.
+__device__ __forceinline__ void tallyWave(unsigned long long* tally, uint32_t n)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(tally, (unsigned long long)n);
+}
+
+// Samples per segment of the unstaged casters' walk when a volume has an occupancy table.
+constexpr int kSegment = 64;
+
+// True if no voxel in the padded-grid box [lo, hi] (inclusive, clamped to the table) is non-zero:
+// eight corner reads of the summed-area table (uniform indices, so scalar loads).
+__device__ __forceinline__ bool bricksEmpty(const VolumeView& vol, const uint32_t vlo[3], const uint32_t vhi[3])
+{
+    uint32_t lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t l = vlo[k] >> kOccShift, h = vhi[k] >> kOccShift, last = vol.occ_n[k] - 1;
+        lo[k] = __builtin_amdgcn_readfirstlane(l < last ? l : last);
+        hi[k] = __builtin_amdgcn_readfirstlane((h < last ? h : last) + 1);
+    }
+    const uint32_t nx = vol.occ_n[0] + 1, nxy = nx * (vol.occ_n[1] + 1);
+    const uint32_t* t = vol.occ;
+    auto at = [&](uint32_t i, uint32_t j, uint32_t k) { return t[k * nxy + j * nx + i]; };
+    const uint32_t n = at(hi[0], hi[1], hi[2]) - at(lo[0], hi[1], hi[2]) - at(hi[0], lo[1], hi[2]) -
+                       at(hi[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) + at(lo[0], hi[1], lo[2]) +
+                       at(hi[0], lo[1], lo[2]) - at(lo[0], lo[1], lo[2]);
+    return n == 0;
+}
+
+// True if the wave's samples with indices in [s0, s1] (0 <= s0 <= s1) read only zero voxels. All
+// rays of a (view, module) start from the same point c, so along each axis the positions
+// fma(i, s, c) of the active rays lie between fma(i, smin, c) and fma(i, smax, c) (the FMA is
+// monotonic in s for i >= 0, and in i), extremes at i = s0 or s1; a sample's taps span
+// floor(q) .. floor(q) + 1, and samples outside [0, n + 1) are never taken.
+__device__ __forceinline__ bool rangeEmpty(const VolumeView& vol, const float nhi[3], int s0, int s1,
+                                           const float smin[3], const float smax[3], const float c[3])
+{
+    uint32_t lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a0 = fmaf(float(s0), smin[k], c[k]), a1 = fmaf(float(s1), smin[k], c[k]);
+        const float b0 = fmaf(float(s0), smax[k], c[k]), b1 = fmaf(float(s1), smax[k], c[k]);
+        const float ql = fmaxf(fminf(a0, a1), 0.0f), qh = fminf(fmaxf(b0, b1), nhi[k]);
+        if (!(ql == ql) || !(qh == qh)) return false; // not a number: take the samples
+        if (qh < ql) return true;                     // the segment's positions miss the support
+        lo[k] = uint32_t(ql);
+        hi[k] = uint32_t(qh) + 1u;
+    }
+    return bricksEmpty(vol, lo, hi);
+}
+
 // LARGE: a volume of 4 GiB or more, read at 64-bit addresses from `base` (interleaved layout
 // only); otherwise through the buffer `rs`.
 template <Storage ST, Layout LY, bool INTERP, bool LARGE = false>
@@ -357,30 +424,53 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                 int a = waveMax(act ? i0 : INT_MIN), bb = waveMin(act ? i1 : INT_MAX);
                 if (a > bb) { a = hi + 1; bb = hi; } // no common range: everything masked
                 const uint32_t span = uint32_t(i1 - i0);
+                // With an occupancy table the range is walked in segments of kSegment samples,
+                // anchored at the common start `a` (kSegment is a multiple of UNROLL, so every sample
+                // lands in the partial sum it lands in without segments), and a segment whose
+                // samples read only zero voxels is skipped: its samples are exactly 0.
+                float smin[3] = { 0.0f, 0.0f, 0.0f }, smax[3] = { 0.0f, 0.0f, 0.0f };
+                if (vol.occ) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        smin[k] = waveMinF(act ? ray.s[k] : FLT_MAX);
+                        smax[k] = waveMaxF(act ? ray.s[k] : -FLT_MAX);
+                    }
+                }
+                const int anchor = a <= hi ? a : lo;
+                const int seg = vol.occ ? kSegment : (hi - lo + 1);
 
                 float part[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
-                // head [lo, a) and, when there is no common range, everything
-                for (int i = lo; i < a; ++i)
-                    if (act && uint32_t(i - i0) <= span)
-                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
-                // body [a, bb]: every active ray samples
-                if (act) {
-                    int i = a;
-                    float fi = float(a);
-                    for (; i + UNROLL - 1 <= bb; i += UNROLL, fi += float(UNROLL)) {
-#pragma unroll
-                        for (int u = 0; u < UNROLL; ++u)
-                            part[u] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi + float(u), ray.s, g.c, vol.data);
+                for (int c0 = anchor - (anchor - lo + seg - 1) / seg * seg; c0 <= hi; c0 += seg) {
+                    const int s0 = max(c0, lo), s1 = min(c0 + seg - 1, hi);
+                    if (vol.occ && rangeEmpty(vol, kc.nhi, s0, s1, smin, smax, g.c)) continue; // uniform
+                    if (kc.tally) {
+                        const int ta = max(s0, i0), te = min(s1, i1);
+                        tallyWave(kc.tally, act && ta <= te ? uint32_t(te - ta + 1) : 0u);
                     }
-                    for (; i <= bb; ++i, fi += 1.0f)
-                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
+                    // head [lo, a) and, when there is no common range, everything
+                    for (int i = s0; i <= min(s1, a - 1); ++i)
+                        if (act && uint32_t(i - i0) <= span)
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
+                    // body [a, bb]: every active ray samples
+                    if (act) {
+                        const int be = min(s1, bb);
+                        int i = max(s0, a);
+                        float fi = float(i);
+                        for (; i + UNROLL - 1 <= be; i += UNROLL, fi += float(UNROLL)) {
+#pragma unroll
+                            for (int u = 0; u < UNROLL; ++u)
+                                part[u] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi + float(u), ray.s, g.c, vol.data);
+                        }
+                        for (; i <= be; ++i, fi += 1.0f)
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
+                    }
+                    // tail (bb, hi]
+                    for (int i = max(s0, bb + 1); i <= s1; ++i)
+                        if (act && uint32_t(i - i0) <= span)
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
                 }
-                // tail (bb, hi]
-                for (int i = bb + 1; i <= hi; ++i)
-                    if (act && uint32_t(i - i0) <= span)
-                        part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
                 float sum = 0.0f;
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) sum += part[u];
@@ -501,6 +591,17 @@ __device__ __forceinline__ void sampleRange(int wlo, int wa, int we, int whi, in
         if (live && uint32_t(i - a) <= span) part[0] += f(float(i));
 }
 
+// True if every voxel that a tap of a chunk with texel box [o, o + nb) can read is exactly 0: the
+// box's voxels with their z + 1 neighbours (held in the texels) and y + 1 (the quads' second row).
+// Every sample of such a chunk is exactly 0, so skipping it leaves each ray's sum unchanged.
+__device__ __forceinline__ bool boxEmpty(const VolumeView& vol, const int o[3], const int nb[3])
+{
+    if (!vol.occ) return false;
+    const uint32_t lo[3] = { uint32_t(o[0]), uint32_t(o[1]), uint32_t(o[2]) };
+    const uint32_t hi[3] = { uint32_t(o[0] + nb[0] - 1), uint32_t(o[1] + nb[1]), uint32_t(o[2] + nb[2]) };
+    return bricksEmpty(vol, lo, hi);
+}
+
 template <Storage ST, int UNROLL, bool LARGE = false>
 __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                                const RayParams& rp, const CastConst& kc,
@@ -590,6 +691,22 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             // with a box to stage (16-bit coordinates valid), 0xffff marks a chunk in which no lane
             // of the workgroup samples; without, sampling below finds no live lane
             if (kc.lds_texels && o[0] == 0xffff) continue; // uniform
+            // a chunk whose box holds only zero voxels adds exactly 0 to every ray (uniform)
+            if (kc.lds_texels && boxEmpty(vol, o, nb)) continue;
+            // so does this wave's part of a chunk whose own box (its lanes' taps) is empty: the wave
+            // then only helps to stage the box for the others (uniform over the wave)
+            bool taken = live;
+            if (kc.lds_texels && vol.occ && (pk[0] & 0xffffu) != 0xffffu) {
+                const int wo[3] = { int(pk[0] & 0xffffu), int(pk[0] >> 16), int(pk[2] & 0xffffu) };
+                const int wn[3] = { int(0xffffu - (pk[1] & 0xffffu)) - wo[0] + 1,
+                                    int(0xffffu - (pk[1] >> 16)) - wo[1] + 1,
+                                    int(0xffffu - (pk[2] >> 16)) - wo[2] + 1 };
+                if (boxEmpty(vol, wo, wn)) {
+                    wlo = INT_MAX; whi = INT_MIN; wa = INT_MIN + 1; we = INT_MIN;
+                    taken = false;
+                }
+            }
+            if (kc.tally) tallyWave(kc.tally, taken ? uint32_t(e - a + 1) : 0u);
             // a large volume stages through a window of the box's slices, addressed with 32 bits
             const bool small = kc.lds_texels && nb[0] >= 1 && nb[0] <= 256 && nb[1] >= 1 && nb[1] <= 256 &&
                                nb[2] >= 1 && nb[2] <= 256 &&
@@ -1153,6 +1270,43 @@ __global__ void sum_kernel(VolumeView s, double* sum)
     if (threadIdx.x == 0) atomicAdd(sum, part[0]);
 }
 
+// Occupancy table: one thread per brick writes 1 if any voxel of its kOccBrick^3 cube of the padded
+// grid is not exactly 0 (NaN included), at table entry (i + 1, j + 1, k + 1); the table's first
+// row, column and slice are 0. Prefix sums along x, then y, then z make it a summed-area table.
+__global__ void occ_flag_kernel(VolumeView v, uint32_t* __restrict__ occ, uint32_t ox, uint32_t oy, uint32_t oz)
+{
+    const size_t total = size_t(ox + 1) * (oy + 1) * (oz + 1);
+    const size_t px = size_t(v.n[0]) + 2, py = size_t(v.n[1]) + 2, pz = size_t(v.n[2]) + 2;
+    for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += size_t(gridDim.x) * blockDim.x) {
+        const size_t i = t % (ox + 1), jk = t / (ox + 1), j = jk % (oy + 1), k = jk / (oy + 1);
+        uint32_t flag = 0;
+        if (i && j && k) {
+            const size_t x0 = (i - 1) << kOccShift, y0 = (j - 1) << kOccShift, z0 = (k - 1) << kOccShift;
+            for (size_t z = z0; z < z0 + kOccBrick && z < pz && !flag; ++z)
+                for (size_t y = y0; y < y0 + kOccBrick && y < py && !flag; ++y)
+                    for (size_t x = x0; x < x0 + kOccBrick && x < px; ++x)
+                        if (!(viewValue(v, x, y, z) == 0.0f)) { flag = 1; break; }
+        }
+        occ[t] = flag;
+    }
+}
+
+// Inclusive prefix sums along one axis of the (ox + 1) x (oy + 1) x (oz + 1) table: one thread per
+// line; `stride` is the distance of neighbours on that axis, `len` the line's length, and the
+// line's start is (line % inner) + (line / inner) * outer.
+__global__ void occ_scan_kernel(uint32_t* __restrict__ occ, size_t lines, size_t inner, size_t outer, size_t stride,
+                                uint32_t len)
+{
+    for (size_t l = size_t(blockIdx.x) * blockDim.x + threadIdx.x; l < lines; l += size_t(gridDim.x) * blockDim.x) {
+        uint32_t* p = occ + (l % inner) + (l / inner) * outer;
+        uint32_t acc = 0;
+        for (uint32_t k = 0; k < len; ++k) {
+            acc += p[k * stride];
+            p[k * stride] = acc;
+        }
+    }
+}
+
 __global__ void zero_kernel(float* d, size_t n)
 {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -1387,11 +1541,12 @@ hipError_t launchSetup(const double* d_pmats, uint32_t views, uint32_t modules, 
 
 hipError_t launchRaycast(const VolumeView& vol, const VolumeGeom& vg, const RayGeom* d_geom,
                          const RayParams& rp, const LaunchShape& shape, float* d_out,
-                         hipStream_t s, int accumulate)
+                         hipStream_t s, int accumulate, unsigned long long* tally)
 {
     if (!validShape(shape)) return hipErrorInvalidValue;
     CastConst kc = makeCastConst(&vol, vg, rp, shape);
     kc.accumulate = accumulate ? 1u : 0u;
+    kc.tally = tally;
     const uint64_t blocks = uint64_t(kc.tiles_per_vm) * rp.views * rp.modules;
     if (blocks == 0) return hipSuccess;
     if (blocks > 0xffffffffull) return hipErrorInvalidConfiguration;
@@ -1432,6 +1587,18 @@ hipError_t launchPack(const float* d_src, const uint32_t n[3], uint32_t f, Stora
     const uint32_t px = n[0] * f + 2, py = n[1] * f + 2, pz = n[2] * f + 2;
     const FromFloat v{ d_src, n[0], n[1], n[2], f };
     build_kernel<<<gridFor(size_t(px) * py * pz), kBlock, 0, s>>>(v, px, py, pz, st, ly, d_dst);
+    return hipGetLastError();
+}
+
+hipError_t launchOccupancy(const VolumeView& view, uint32_t* d_occ, hipStream_t s)
+{
+    const size_t ox = occBricks(view.n[0]), oy = occBricks(view.n[1]), oz = occBricks(view.n[2]);
+    const size_t nx = ox + 1, ny = oy + 1, nz = oz + 1;
+    occ_flag_kernel<<<gridFor(nx * ny * nz), kBlock, 0, s>>>(view, d_occ, uint32_t(ox), uint32_t(oy), uint32_t(oz));
+    // x lines: start (j, k) = l * nx; y lines: start i + k * nx * ny; z lines: start i + j * nx
+    occ_scan_kernel<<<gridFor(ny * nz), kBlock, 0, s>>>(d_occ, ny * nz, 1, nx, 1, uint32_t(nx));
+    occ_scan_kernel<<<gridFor(nx * nz), kBlock, 0, s>>>(d_occ, nx * nz, nx, nx * ny, nx, uint32_t(ny));
+    occ_scan_kernel<<<gridFor(nx * ny), kBlock, 0, s>>>(d_occ, nx * ny, nx * ny, 0, nx * ny, uint32_t(nz));
     return hipGetLastError();
 }
 

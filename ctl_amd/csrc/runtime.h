@@ -24,6 +24,7 @@ struct ctlproj_volume {
     ctlproj_context* ctx = nullptr;
     void* d_data = nullptr;
     size_t bytes = 0;
+    uint32_t* d_occ = nullptr; // occupancy table of view.occ (NULL: none)
     ctlk::VolumeView view{};
     uint32_t n[3] = { 0, 0, 0 };
     float voxel_mm[3] = { 0, 0, 0 };

@@ -567,6 +567,16 @@ class RayCasterProjector:
         check(lib().ctlproj_projector_count_samples(self.h, volume.h, C.byref(n)))
         return n.value
 
+    def count_taken(self, volume: ResidentVolume, device_out: int, out_count: int) -> int:
+        """Projects into device_out and returns the number of samples the kernels took
+        (ctlproj_projector_count_taken): the reference's count less the samples of chunks that
+        read only zero voxels, which are skipped."""
+        self._push_settings()
+        n = C.c_uint64()
+        check(lib().ctlproj_projector_count_taken(self.h, volume.h, C.c_void_p(device_out), out_count,
+                                                  C.byref(n)))
+        return n.value
+
     def gather_probe(self, volume: ResidentVolume, mode: str, device_out: int, out_count: int):
         """Gather-ceiling diagnostic (ctlproj_diag_gather_probe): mode "gather" issues this
         projection's texel loads without the interpolation, mode "stream" reads a 16 KiB

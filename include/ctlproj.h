@@ -95,7 +95,12 @@ ctlproj_status ctlproj_context_destroy(ctlproj_context* ctx);
 /* ---- resident volumes (replaces the per-call Image3D upload, raycasterprojector.cpp:236-251) */
 
 /* Uploads `host` (float, X*Y*Z) and keeps it resident in HBM in `storage` precision.
- * voxel_mm: voxel size, offset_mm: volume offset (VoxelVolume::offset). */
+ * voxel_mm: voxel size, offset_mm: volume offset (VoxelVolume::offset). Volumes of 4 GiB or more
+ * (padded 8-byte texels) are projected but refused by batches, the server and volume processing.
+ * The volume also keeps an occupancy table of its 4x4x4-voxel bricks (a summed-area table, about
+ * 1/64 of the voxel count in 4-byte entries), with which the ray casters skip chunks of samples
+ * that read only zero voxels; the environment variable CTLPROJ_EMPTY_SKIP=0 at creation leaves it
+ * out. */
 ctlproj_status ctlproj_volume_create(ctlproj_context* ctx, const float* host, const uint32_t dims[3],
                                      const float voxel_mm[3], const float offset_mm[3],
                                      ctlproj_dtype storage, ctlproj_volume** out);
@@ -212,6 +217,13 @@ ctlproj_status ctlproj_projector_last_kernel_ms(ctlproj_projector* p, float* ms)
  * (sum over rays of floor(lambda_max) + 2 - floor(lambda_min)), counted on the device. */
 ctlproj_status ctlproj_projector_count_samples(ctlproj_projector* p, const ctlproj_volume* vol,
                                                uint64_t* n_samples);
+
+/* Diagnostics (no reference counterpart): projects all views into device_out, as
+ * ctlproj_project_device does, and returns the number of samples the kernels actually took. The
+ * casters skip chunks of samples that read only zero voxels (exactly 0, so every output is the
+ * same as with all samples taken); n_taken excludes them. */
+ctlproj_status ctlproj_projector_count_taken(ctlproj_projector* p, const ctlproj_volume* vol, float* device_out,
+                                            size_t out_count, uint64_t* n_taken);
 
 /* Measures the candidate launch shapes (wavefront pixel tile, ray-step unroll, LDS chunk and box;
  * for narrow modules also with the wavefronts stacked) on up to 24 of the configured views and

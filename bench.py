@@ -34,9 +34,11 @@ ONCHIP_PEAK_GBPS = L1_PEAK_GBPS + LDS_PEAK_GBPS
 # HBM traffic per launch of the 360-view FP32 projection (LDS-staged kernel), from the rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE passes in profiles/staged_kernel_f32_256cube_360views_1024x768.md, in
 # KiB per dispatch, each scaled by what the counter reads for a plain 1 GiB device copy
-# (bench.py --copy-calibration: FETCH_SIZE 524300 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
-FETCH_KIB, FETCH_CAL = 5.1107e7, 1048576 / 524300
-WRITE_KIB, WRITE_CAL = 1.1767e6, 1.0
+# (bench.py --copy-calibration: FETCH_SIZE 524315 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
+# Chunks that read only zero voxels are skipped, so the kernel fetches about half the bytes it
+# fetches with every sample taken (5.11e7 KiB).
+FETCH_KIB, FETCH_CAL = 2.6067e7, 1048576 / 524315
+WRITE_KIB, WRITE_CAL = 1.1082e6, 1.0
 TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
 
 
@@ -73,6 +75,11 @@ def bench_real(args, torch, W, ctl, oracle):
         vol = ctl.ResidentVolume(ctl.VoxelVolume(w.volume, w.voxel_mm, w.offset_mm), storage)
         n_samples = p.count_samples(vol)
         wall, kms = timed_projection(p, vol, out, args.steps, args.warmup)
+        if args.kernels_only:  # profiling: the timed launches and nothing else of this kernel
+            res[storage] = dict(ms_per_step=wall * 1e3, kernel_ms=kms, n_samples=n_samples)
+            log(f"real {storage}: {json.dumps(res[storage])}")
+            del vol
+            continue
         # samples the kernels take: chunks that read only zero voxels (60% of this phantom's
         # voxels are 0) are skipped with an identical result
         n_taken = p.count_taken(vol, out.data_ptr(), out.numel())
@@ -93,10 +100,6 @@ def bench_real(args, torch, W, ctl, oracle):
                                            gsamples_s=n_samples / dwall / 1e9,
                                            gather_GBps=n_samples * 8 * eb / (dkms * 1e-3) / 1e9)
         del dense
-        if args.kernels_only:
-            log(f"real {storage}: {json.dumps(res[storage])}")
-            del vol
-            continue
         # gather ceiling of this access pattern: the same loads without the interpolation
         pg = [p.gather_probe(vol, "gather", out.data_ptr(), out.numel()) for _ in range(4)][1:]
         ceil = float(np.median([b / (ms * 1e-3) / 1e9 for ms, b in pg]))
@@ -153,6 +156,8 @@ def bench_helical(args, torch, W, ctl, oracle):
         wall, kms = timed_projection(p, vol, out, max(1, args.steps // 2), 1)
         res[storage] = dict(ms_per_step=wall * 1e3, kernel_ms=kms, n_samples=n_samples,
                             gsamples_s=n_samples / wall / 1e9, views_s=w.views / wall)
+        if not args.kernels_only:
+            res[storage]["n_taken"] = p.count_taken(vol, out.data_ptr(), out.numel())
         log(f"helical {storage}: {json.dumps(res[storage])}")
         if not args.no_errors:
             pick = [0, 1234]
@@ -449,6 +454,9 @@ def main() -> int:
         print(json.dumps({"helical": bench_helical(args, torch, W, ctl, oracle)}), flush=True)
         return 0
     w, real = bench_real(args, torch, W, ctl, oracle)
+    if args.kernels_only:
+        print(json.dumps({"real": real}), flush=True)
+        return 0
     extra = {}
     if not args.only_real:
         extra["helical"] = bench_helical(args, torch, W, ctl, oracle)

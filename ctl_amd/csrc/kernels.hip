@@ -184,13 +184,14 @@ __device__ __forceinline__ void checkFace(float h0, float h1, float c10, float c
 // position frame c = source - corner + shift, computed once per view) the sample positions are
 // the reference's bit for bit. nhi = n + 1 per axis; blo = 0 selects the trilinear box (margin
 // 1/2), blo = 1 the nearest-mode box (no margin, raycasterprojector_no_interp.cl:95-140).
-__device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, float inc,
+template <class G> // RayGeom, or RayGeom in the constant address space
+__device__ __forceinline__ Ray setupRay(const G& g, float px, float py, float inc,
                                         const float vs[3], const float nhi[3], float blo)
 {
 #pragma clang fp contract(off)
     Ray r;
     const double x = double(px), y = double(py);
-    const double* q = g.qr;
+    const auto* q = g.qr;
     const double qx = q[0] * x + q[3] * y + q[6];
     const double qy = q[1] * x + q[4] * y + q[7];
     const double qz = q[2] * x + q[5] * y + q[8];
@@ -214,7 +215,7 @@ __device__ __forceinline__ Ray setupRay(const RayGeom& g, float px, float py, fl
         c2[k] += 0.01f;
     }
     float lo = FLT_MAX, hi = 0.0f;
-    const float* c = g.src;
+    const auto* c = g.src;
     const float* s = r.s;
     checkFace(c[1] + l1[0] * s[1], c[2] + l1[0] * s[2], c1[1], c1[2], c2[1], c2[2], l1[0], lo, hi);
     checkFace(c[0] + l1[1] * s[0], c[2] + l1[1] * s[2], c1[0], c1[2], c2[0], c2[2], l1[1], lo, hi);
@@ -316,7 +317,10 @@ __device__ __forceinline__ bool bricksEmpty(const VolumeView& vol, const uint32_
         hi[k] = __builtin_amdgcn_readfirstlane((h < last ? h : last) + 1);
     }
     const uint32_t nx = vol.occ_n[0] + 1, nxy = nx * (vol.occ_n[1] + 1);
-    const uint32_t* t = vol.occ;
+    // read as constant memory (the table is not written while kernels run): with uniform indices
+    // these are scalar loads, which leave the texture path to the gathers
+    typedef __attribute__((address_space(4))) const uint32_t occ_word;
+    const occ_word* t = (const occ_word*)(vol.occ);
     auto at = [&](uint32_t i, uint32_t j, uint32_t k) { return t[k * nxy + j * nx + i]; };
     const uint32_t n = at(hi[0], hi[1], hi[2]) - at(lo[0], hi[1], hi[2]) - at(hi[0], lo[1], hi[2]) -
                        at(hi[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) + at(lo[0], hi[1], lo[2]) +
@@ -412,11 +416,16 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                 // at a time (served FP16 scans 1.6x slower)
                 const RayGeom* gr = gp;
                 asm volatile("" : "+s"(gr));
-                const RayGeom& g = *gr;
+                // read as constant memory (nothing writes it while kernels run): scalar loads that
+                // leave the texture path to the gathers; the sample positions' origin goes to
+                // registers once
+                typedef __attribute__((address_space(4))) const RayGeom geom_const;
+                const geom_const& g = *(const geom_const*)gr;
+                const float gc[3] = { g.c[0], g.c[1], g.c[2] };
                 Ray ray = setupRay(g, subRayCoord(x, ipx, ra), subRayCoord(y, ipy, rb), kc.inc, kc.vs,
                                    kc.nhi, kc.blo);
                 if (!pixel) ray.hit = false;
-                tighten(ray, g.c, kc.blo, kc.nhi);
+                tighten(ray, gc, kc.blo, kc.nhi);
                 const bool act = pixel && ray.i0 <= ray.i1;
                 const int i0 = act ? ray.i0 : INT_MAX, i1 = act ? ray.i1 : INT_MIN;
                 const int lo = waveMin(i0), hi = waveMax(i1);
@@ -444,7 +453,7 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                 for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
                 for (int c0 = anchor - (anchor - lo + seg - 1) / seg * seg; c0 <= hi; c0 += seg) {
                     const int s0 = max(c0, lo), s1 = min(c0 + seg - 1, hi);
-                    if (vol.occ && rangeEmpty(vol, kc.nhi, s0, s1, smin, smax, g.c)) continue; // uniform
+                    if (vol.occ && rangeEmpty(vol, kc.nhi, s0, s1, smin, smax, gc)) continue; // uniform
                     if (kc.tally) {
                         const int ta = max(s0, i0), te = min(s1, i1);
                         tallyWave(kc.tally, act && ta <= te ? uint32_t(te - ta + 1) : 0u);
@@ -452,7 +461,7 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                     // head [lo, a) and, when there is no common range, everything
                     for (int i = s0; i <= min(s1, a - 1); ++i)
                         if (act && uint32_t(i - i0) <= span)
-                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, gc, vol.data);
                     // body [a, bb]: every active ray samples
                     if (act) {
                         const int be = min(s1, bb);
@@ -461,15 +470,15 @@ __device__ __forceinline__ void castTile(const VolumeView& vol, const RayGeom* _
                         for (; i + UNROLL - 1 <= be; i += UNROLL, fi += float(UNROLL)) {
 #pragma unroll
                             for (int u = 0; u < UNROLL; ++u)
-                                part[u] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi + float(u), ray.s, g.c, vol.data);
+                                part[u] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi + float(u), ray.s, gc, vol.data);
                         }
                         for (; i <= be; ++i, fi += 1.0f)
-                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, fi, ray.s, gc, vol.data);
                     }
                     // tail (bb, hi]
                     for (int i = max(s0, bb + 1); i <= s1; ++i)
                         if (act && uint32_t(i - i0) <= span)
-                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, g.c, vol.data);
+                            part[0] += sampleAt<ST, LY, INTERP, LARGE>(rs, kc.pitch, float(i), ray.s, gc, vol.data);
                 }
                 float sum = 0.0f;
 #pragma unroll
@@ -618,14 +627,18 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
     const uint32_t y = (ty * kc.bwy + wy) * kc.wave_h + ly;
     const bool pixel = x < rp.channels && y < rp.rows;
 
-    const RayGeom g = geom[vm];
+    // the geometry record read as constant memory (scalar loads; in the batch kernel the record's
+    // address comes from the item table, where the compiler would otherwise use vector loads)
+    typedef __attribute__((address_space(4))) const RayGeom geom_const;
+    const geom_const& g = *(const geom_const*)(geom + vm);
+    const float gc[3] = { g.c[0], g.c[1], g.c[2] };
     const __amdgpu_buffer_rsrc_t rs = makeRsrc(vol.data, vol.bytes); // unused by LARGE gathers
     float acc = 0.0f;
     if (g.valid) { // uniform over the workgroup: one (view, module) per workgroup
         // one ray per pixel: the sub-ray coordinate x - 1/2 + 1/2 is x exactly
         Ray ray = setupRay(g, float(x), float(y), kc.inc, kc.vs, kc.nhi, kc.blo);
         if (!pixel) ray.hit = false;
-        tighten(ray, g.c, kc.blo, kc.nhi);
+        tighten(ray, gc, kc.blo, kc.nhi);
         const bool act = pixel && ray.i0 <= ray.i1;
         // sample ranges, once per tile: the union [r_lo, r_hi] and the intersection [r_alo, r_ahi]
         // of the wave's rays, and the workgroup's union. Clipped to a chunk, the union covers
@@ -659,7 +672,7 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                 uint32_t lo[3], hi[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float qa = fmaf(float(as), ray.s[k], g.c[k]), qe = fmaf(float(es), ray.s[k], g.c[k]);
+                    const float qa = fmaf(float(as), ray.s[k], gc[k]), qe = fmaf(float(es), ray.s[k], gc[k]);
                     const uint32_t extra = (k == 0 || (k == 1 && ST == Storage::F32)) ? 1u : 0u;
                     lo[k] = uint32_t(int(fminf(qa, qe))) & 0xffffu;
                     hi[k] = 0xffffu - ((uint32_t(int(fmaxf(qa, qe))) + extra) & 0xffffu);
@@ -768,9 +781,9 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                 const uint32_t px8 = uint32_t(px) * 8u, pz8 = uint32_t(pz) * 8u;
                 const uint32_t base8 = (uint32_t(o[0]) + uint32_t(o[1]) * uint32_t(px) + uint32_t(o[2]) * uint32_t(pz)) * 8u;
                 sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, [&](float fi) {
-                    const float qx = fmaf(fi, ray.s[0], g.c[0]);
-                    const float qy = fmaf(fi, ray.s[1], g.c[1]);
-                    const float qz = fmaf(fi, ray.s[2], g.c[2]);
+                    const float qx = fmaf(fi, ray.s[0], gc[0]);
+                    const float qy = fmaf(fi, ray.s[1], gc[1]);
+                    const float qz = fmaf(fi, ray.s[2], gc[2]);
                     const uint32_t ofs = __umul24(uint32_t(int(qz)), pz8) +
                                          (__umul24(uint32_t(int(qy)), px8) + ((uint32_t(int(qx)) << 3) - base8));
                     const float wx = __builtin_amdgcn_fractf(qx), wy = __builtin_amdgcn_fractf(qy);
@@ -782,11 +795,11 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                     // vector L1 instead (the box was just read from there): the LDS pipe and the
                     // texture path then share the taps. 256^3 / 1024 x 768, 90 views: a third
                     // 19.0 ms, a quarter 19.5, half 21.1, none 22.7
-                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
+                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, gc, vol.data);
                 });
             } else {
                 auto gather = [&](float fi) {
-                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, g.c, vol.data);
+                    return sampleAt<ST, Layout::Interleaved, true, LARGE>(rs, kc.pitch, fi, ray.s, gc, vol.data);
                 };
                 sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, gather, gather);
             }

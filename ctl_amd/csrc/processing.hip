@@ -106,6 +106,12 @@ __global__ __launch_bounds__(kBlock) void sample_kernel(VolumeView vol, Ranges r
                                  r.scale[2] * (p[2] - r.lo[2]) + 0.5f);
 }
 
+// A volume's occupancy table (VolumeView::occ, occ_n) for the plane kernel; t NULL: none.
+struct OccTable {
+    const uint32_t* t;
+    uint32_t n[3];
+};
+
 // Reads of the plane kernel: the resident volume's own texels, or its FP32 brick copy.
 template <Storage ST, Layout LY> struct TexelFetch {
     VolumeView vol;
@@ -229,12 +235,16 @@ __global__ __launch_bounds__(kBlock) void pair16_kernel(VolumeView src, uint32_t
 // together and the group's rows one after the other, so tiles sharing texel rows are read close
 // together in time. A wavefront skips a tile when the box spanned by its corner pixels and the
 // chunk's distance range misses the volume's support on some axis (with half a texel to spare):
-// every sample of such a tile reads the zero border.
+// every sample of such a tile reads the zero border. With the volume's occupancy table, a wavefront
+// also skips the distances of a tile whose samples read only bricks that hold no non-zero voxel:
+// lane 8 j + c reads corner c of the summed-area table for distance j (one gather for all eight
+// distances), and the eight lanes of a distance add their signed corners. Skipped samples are
+// exactly 0, so every sum is the one without skipping, bit for bit.
 template <class FETCH>
 __global__ __launch_bounds__(kBlock) void plane_kernel(FETCH fetch, uint32_t n0, uint32_t n1, uint32_t n2,
                                                        const PlaneMap* __restrict__ maps,
                                                        const float* __restrict__ dist, PlaneJob job,
-                                                       float* __restrict__ partial)
+                                                       OccTable occ, float* __restrict__ partial)
 {
     const uint32_t b = xcdOrder(blockIdx.x, gridDim.x);
     const uint32_t split = b % job.splits, rest = b / job.splits;
@@ -270,6 +280,14 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(FETCH fetch, uint32_t n0,
         ehi[k] = h[k][2] + fmaxf(dlo * s[k], dhi * s[k]);
     }
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the occupancy test's distance (lane / 8) and table corner (lane % 8: bit k picks the upper
+    // bound on axis k), and that distance's constant part of each texel coordinate
+    const uint32_t cj = lane >> 3, cc = lane & 7;
+    const float cd = dist[d0 + (cj < nd ? cj : 0u)];
+    float ce[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ce[k] = fmaf(cd, s[k], h[k][2]);
+    const uint32_t all = nd >= 32 ? ~0u : (1u << nd) - 1u;
     for (uint32_t v = v0 + wave; v < v1; v += kBlock / 64) {
         const uint32_t g = v >> 4, w = v & 15;
         const uint32_t tx = (g % groups_x) * 4 + (w & 3), ty = (g / groups_x) * 4 + (w >> 2);
@@ -284,6 +302,36 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(FETCH fetch, uint32_t n0,
             miss = miss || hi < -0.5f || lo >= nhi[k] + 0.5f;
         }
         if (miss) continue; // uniform: the tile's values are the same for every lane
+        uint32_t live = all; // distances of the chunk whose samples may read a non-zero voxel
+        if (occ.t) {
+            bool empty = cj >= nd;
+            uint32_t idx[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                // positions of this distance's samples in the tile, with half a texel to spare; a
+                // sample at q reads voxels floor(q) and floor(q) + 1, and only samples in [0, n + 1)
+                // read any
+                const float ax = h[k][0] * x0, bx = h[k][0] * x1, ay = h[k][1] * y0, by = h[k][1] * y1;
+                const float lo = fmaxf(ce[k] + fminf(ax, bx) + fminf(ay, by) - 0.5f, 0.0f);
+                const float hi = fminf(ce[k] + fmaxf(ax, bx) + fmaxf(ay, by) + 0.5f, nhi[k]);
+                empty = empty || !(lo <= hi);
+                const uint32_t last = occ.n[k] - 1u;
+                const uint32_t bl = min(uint32_t(lo) >> kOccShift, last);
+                const uint32_t bh = min((uint32_t(hi) + 1u) >> kOccShift, last) + 1u;
+                idx[k] = ((cc >> k) & 1u) ? bh : bl;
+            }
+            const uint32_t nx = occ.n[0] + 1u, nxy = nx * (occ.n[1] + 1u);
+            int cnt = int(occ.t[idx[2] * nxy + idx[1] * nx + idx[0]]);
+            cnt = (__popc(~cc & 7u) & 1u) ? -cnt : cnt; // summed-area corner sign
+            cnt += __shfl_xor(cnt, 1, 64);
+            cnt += __shfl_xor(cnt, 2, 64);
+            cnt += __shfl_xor(cnt, 4, 64);
+            const uint64_t bal = __ballot(cc == 0 && !empty && cnt != 0);
+            live = 0;
+#pragma unroll
+            for (int j = 0; j < kPlaneDists; ++j) live |= uint32_t((bal >> (8 * j)) & 1u) << j;
+            if (!live) continue; // uniform
+        }
         const uint32_t x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
         if (x >= job.width || y >= job.height) continue;
         const float fx = float(x), fy = float(y);
@@ -294,7 +342,8 @@ __global__ __launch_bounds__(kBlock) void plane_kernel(FETCH fetch, uint32_t n0,
         // drops those sums), with no branch between the samples
 #pragma unroll
         for (int j = 0; j < kPlaneDists; ++j)
-            acc[j] += fetch(nhi, fmaf(dd[j], s[0], b0), fmaf(dd[j], s[1], b1), fmaf(dd[j], s[2], b2));
+            if ((live >> j) & 1u) // uniform
+                acc[j] += fetch(nhi, fmaf(dd[j], s[0], b0), fmaf(dd[j], s[1], b1), fmaf(dd[j], s[2], b2));
     }
     __shared__ float red[kBlock / 64][kPlaneDists];
 #pragma unroll
@@ -439,6 +488,7 @@ hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, Bri
     if (blocks > 0x7fffffffull || uint64_t(job.n_orient) * job.n_dist > 0xffffffffull - kBlock)
         return hipErrorInvalidConfiguration;
     const uint32_t n0 = vol.n[0], n1 = vol.n[1], n2 = vol.n[2];
+    const OccTable oc{ vol.occ, { vol.occ_n[0], vol.occ_n[1], vol.occ_n[2] } };
     if (d_brick) {
         uint32_t nb[3];
         size_t bytes = 0;
@@ -446,25 +496,25 @@ hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, Bri
         if (!brickShape(vol.n, brick_kind, &k, nb, &bytes) || k != brick_kind) return hipErrorInvalidValue;
         if (k == BrickKind::Texel8) {
             const Texel8Fetch f{ d_brick, uint32_t(bytes), nb[0] * 128u, nb[0] * nb[1] * 128u };
-            plane_kernel<Texel8Fetch><<<uint32_t(blocks), kBlock, 0, s>>>(f, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            plane_kernel<Texel8Fetch><<<uint32_t(blocks), kBlock, 0, s>>>(f, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
         } else {
             const Pair16Fetch f{ d_brick, uint32_t(bytes), nb[0] * 128u, nb[0] * nb[1] * 128u };
-            plane_kernel<Pair16Fetch><<<uint32_t(blocks), kBlock, 0, s>>>(f, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            plane_kernel<Pair16Fetch><<<uint32_t(blocks), kBlock, 0, s>>>(f, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
         }
     } else {
         const bool il = vol.layout == Layout::Interleaved;
         switch (vol.storage) {
         case Storage::F16:
-            if (il) plane_kernel<TexelFetch<Storage::F16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
-            else plane_kernel<TexelFetch<Storage::F16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            if (il) plane_kernel<TexelFetch<Storage::F16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
+            else plane_kernel<TexelFetch<Storage::F16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
             break;
         case Storage::BF16:
-            if (il) plane_kernel<TexelFetch<Storage::BF16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
-            else plane_kernel<TexelFetch<Storage::BF16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            if (il) plane_kernel<TexelFetch<Storage::BF16, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
+            else plane_kernel<TexelFetch<Storage::BF16, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
             break;
         default:
-            if (il) plane_kernel<TexelFetch<Storage::F32, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
-            else plane_kernel<TexelFetch<Storage::F32, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, d_partial);
+            if (il) plane_kernel<TexelFetch<Storage::F32, Layout::Interleaved>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
+            else plane_kernel<TexelFetch<Storage::F32, Layout::Plain>><<<uint32_t(blocks), kBlock, 0, s>>>({ vol }, n0, n1, n2, d_maps, d_dist, job, oc, d_partial);
             break;
         }
     }

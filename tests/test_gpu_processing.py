@@ -210,6 +210,43 @@ def test_plane_integrals_brick_kind_switch_on_one_volume(monkeypatch):
         assert np.array_equal(got, texels)
 
 
+@pytest.mark.parametrize("storage", STORAGES)
+@pytest.mark.parametrize("reads", ["bricks", "pairs", "texels"])
+def test_plane_integrals_skip_empty_space_exactly(storage, reads, monkeypatch):
+    """The plane kernel skips the distances of a tile whose samples read only empty 4^3-voxel
+    bricks of the volume's occupancy table. Those samples are exactly 0: on a phantom with empty
+    space, and on a volume of a few non-zero voxels (at the grid's corners too), the integrals
+    equal the ones taken over a volume without a table (CTLPROJ_EMPTY_SKIP=0) bit for bit, and the
+    oracle."""
+    from ctl_amd import workloads as W
+    rng = np.random.default_rng(41)
+    sparse = np.zeros((27, 31, 23), np.float32)
+    for x, y, z in [(0, 0, 0), (26, 30, 22), (0, 30, 0), (26, 0, 22)] + [
+            tuple(int(rng.integers(0, n)) for n in sparse.shape) for _ in range(6)]:
+        sparse[x, y, z] = rng.uniform(0.5, 1.0)
+    phantom = W.small(views=1).volume[::2, ::2, ::2].copy()
+    assert (phantom == 0).mean() > 0.3
+    if reads != "bricks":
+        monkeypatch.setenv("CTLPROJ_PLANE_READS", reads)
+    normals = np.vstack([np.eye(3), random_normals(rng, 9)])
+    for data, voxel, offset in ((sparse, (0.9, 1.1, 0.8), (1.5, -2.0, 0.7)), (phantom, (2.0, 2.0, 2.0), (0, 0, 0))):
+        vol = VoxelVolume(np.ascontiguousarray(data.transpose(2, 1, 0)), voxel, offset)
+        ext = max(n * v for n, v in zip(vol.dims, voxel))
+        dist = np.concatenate([rng.uniform(-0.6 * ext, 0.6 * ext, 22), [0.0, 2 * ext]]).astype(np.float32)
+        monkeypatch.setenv("CTLPROJ_EMPTY_SKIP", "1")
+        on = RadonTransform3D(vol, storage)
+        monkeypatch.setenv("CTLPROJ_EMPTY_SKIP", "0")
+        off = RadonTransform3D(vol, storage)
+        monkeypatch.delenv("CTLPROJ_EMPTY_SKIP")
+        got, want = on.planeIntegrals(normals, dist), off.planeIntegrals(normals, dist)
+        assert np.array_equal(got, want)
+        side, reso = on.sliceDimensions()[0], on.sliceResolution()
+        r32 = oracle.plane_integrals(rounded(vol.data, storage), vol.voxel_size, vol.offset, normals, dist, side,
+                                     side, reso)
+        assert np.abs(got - r32).max() <= 2e-5 * np.abs(r32).max()
+        assert np.count_nonzero(got) > 0
+
+
 def test_plane_integrals_axis_layer_sums_and_single_integral():
     rng = np.random.default_rng(11)
     X, Y, Z = 10, 12, 6

@@ -324,12 +324,13 @@ def bench_processing(args, torch, w, ctl, oracle):
         orientations=len(normals), distances=len(dist), side=side, resolution_mm=reso, ms=wall * 1e3,
         template_samples=samples, template_gsamples_s=samples / wall / 1e9,
         inside_samples=inside, inside_gsamples_s=inside / wall / 1e9,
-        roofline={"bound": "l1", "achieved": gather, "peak": L1_PEAK_GBPS, "unit": "GB/s",
-                  "frac": gather / L1_PEAK_GBPS,
-                  "note": "achieved = samples inside the support x 8 taps x 4 B per wall second; "
-                          "the kernel is bound by the vector-L1 line rate of its gathers together "
-                          "with the last-level cache its brick copy is read from (counters in "
-                          "profiles/plane_kernel_f32_256cube_1024orient_128dist.md)"})}
+        roofline={"bound": "lds", "achieved": gather, "peak": LDS_PEAK_GBPS, "unit": "GB/s",
+                  "frac": gather / LDS_PEAK_GBPS, "frac_of_l1_peak": gather / L1_PEAK_GBPS,
+                  "note": "achieved = samples inside the support x 8 taps x 4 B per wall second "
+                          "(samples of empty bricks are skipped with the same sums); the brick "
+                          "kernel reads its taps from bricks of 16^3 cells staged in LDS (two "
+                          "ds_read2_b64 per sample) and is bound by instruction issue and LDS bank "
+                          "conflicts (counters in profiles/plane_brick_kernel_f32_256cube_1024orient_128dist.md)"})}
     log(f"radon3d: {json.dumps(res['radon3d'])}")
     del ones
     if oracle is not None and not args.no_cpu:

@@ -289,6 +289,14 @@ struct alignas(16) PlaneMap {
 };
 static_assert(sizeof(PlaneMap) == 64, "PlaneMap is one 64-byte record");
 
+// Inverse of a plane map's linear part: rows give the template pixel x, y and the distance d of a
+// texel position q as m[r] . (q - h2), h2 the map's constant column plus 1/2.
+struct alignas(16) PlaneInv {
+    float m[3][3];
+    float pad[3];
+};
+static_assert(sizeof(PlaneInv) == 48, "PlaneInv is one 48-byte record");
+
 // Distances per workgroup of the plane-integral kernel.
 constexpr int kPlaneDists = 8;
 
@@ -329,5 +337,19 @@ hipError_t launchBrick(const VolumeView& vol, BrickKind kind, float* d_dst, hipS
 hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, BrickKind brick_kind,
                                 const PlaneMap* d_maps, const float* d_dist, const PlaneJob& job, float* d_partial,
                                 float scale, float* d_out, uint32_t out_stride, hipStream_t s);
+
+// Plane integrals by volume bricks staged in LDS (needs the volume's brick copy): n_orient x n_dist
+// integrals into d_out[d * out_stride + o], accumulated in d_acc (n_orient * n_dist 64-bit
+// fixed-point totals, 2^fix_shift units per value; every |total| must stay below 2^62) by
+// workgroups of one brick and o_chunk orientations. planeBricks() is the number of bricks.
+uint32_t planeBricks(const uint32_t n[3]);
+hipError_t launchPlaneBricks(const VolumeView& vol, const float* d_brick, BrickKind brick_kind, const PlaneMap* d_maps,
+                             const PlaneInv* d_inv, const float* d_dist, uint32_t n_orient, uint32_t n_dist,
+                             uint32_t width, uint32_t height, uint32_t o_chunk, int fix_shift,
+                             unsigned long long* d_acc, float scale, float* d_out, uint32_t out_stride,
+                             hipStream_t s);
+// Largest |v| of a brick copy of `bytes` bytes, as float bits, atomically maxed into *d_max (which
+// the caller clears); NaN and infinity give bits >= 0x7f800000.
+hipError_t launchBrickMaxAbs(const float* d_brick, size_t bytes, uint32_t* d_max, hipStream_t s);
 
 } // namespace ctlk

@@ -3,7 +3,12 @@
 //   slice_kernel     one thread per slice pixel of any number of planes (VolumeSlicer).
 //   resample_kernel  one thread per grid point, sample_kernel one per listed point
 //                    (VolumeResampler).
-//   plane_kernel     plane integrals (RadonTransform3D): a workgroup takes one plane orientation,
+//   plane_brick_kernel  plane integrals (RadonTransform3D), the default: a workgroup stages one
+//                    brick of the volume in LDS and sums, for many orientations, the samples of
+//                    every plane through it (details at the kernel); plane_fix_kernel converts its
+//                    fixed-point totals.
+//   plane_kernel     plane integrals by tiles (volumes without a brick copy or with non-finite
+//                    voxels): a workgroup takes one plane orientation,
 //                    up to kPlaneDists distances and a range of 8x8 pixel tiles of the template
 //                    plane; every wavefront walks its tiles, reading all distances of a pixel
 //                    back to back (the same small patch of the volume, so the gathers share L1
@@ -17,6 +22,8 @@
 #include "sampler.h"
 
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 
 namespace ctlk {
 
@@ -373,6 +380,249 @@ __global__ __launch_bounds__(kBlock) void plane_reduce_kernel(const float* __res
     out[size_t(d) * out_stride + o] = s * scale;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Plane integrals by volume bricks (the default when the volume has an FP32 brick copy).
+//
+// A workgroup takes one brick of kPlaneBrick^3 cells of the padded texel frame (a sample at q
+// belongs to the cell floor(q)) and a range of orientations. It copies the brick's texels (the
+// cells' taps: x and y one further, z carried by the (v, dz) texels) from the brick copy into
+// LDS once and then, for every orientation and every distance whose plane cuts the brick, sums
+// the samples of that plane whose cell lies in the brick: the template pixels of the plane's
+// intersection polygon with the brick, walked in 8 x 8-pixel tiles of its bounding rectangle.
+// Every template sample inside the support belongs to exactly one brick (its position is computed
+// with the same FP32 operations as in plane_kernel, so the same cell), and a brick without a
+// non-zero voxel (occupancy table) is skipped. One workgroup thus serves many orientations from one
+// staging of its voxels instead of every orientation streaming the volume through the caches.
+//
+// Each (brick, orientation, distance) sum is added to its plane's total as a 64-bit fixed-point
+// integer (2^fix_shift units per value), so the totals do not depend on the order in which
+// workgroups finish; plane_fix_kernel converts them back. The host picks fix_shift from the
+// volume's largest |value| so that no total can overflow.
+constexpr int kPlaneBrick = 16;
+constexpr int kPBX = kPlaneBrick + 1, kPBY = kPlaneBrick + 1, kPBZ = kPlaneBrick;
+constexpr int kPBTexels = kPBX * kPBY * kPBZ; // 4624 texels, 36,992 bytes of LDS
+
+// Texel (x, y, z) of the padded frame, (v, dz), from a brick copy.
+struct Texel8Src {
+    const float* data;
+    uint32_t bytes, row, slab;
+    __device__ __forceinline__ float2 operator()(uint32_t x, uint32_t y, uint32_t z) const
+    {
+        const uint32_t o = (z >> 1) * slab + (y >> 1) * row + ((x >> 2) << 7) + ((z & 1u) << 6) + ((y & 1u) << 5) +
+                           ((x & 3u) << 3);
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(makeRsrc(data, bytes), o, 0, 0));
+    }
+};
+struct Pair16Src {
+    const float* data;
+    uint32_t bytes, row, slab;
+    __device__ __forceinline__ float2 operator()(uint32_t x, uint32_t y, uint32_t z) const
+    {
+        const uint32_t o = (z >> 1) * slab + (y >> 1) * row + ((x >> 1) << 7) + ((z & 1u) << 6) + ((y & 1u) << 5) +
+                           ((x & 1u) << 4);
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(makeRsrc(data, bytes), o, 0, 0));
+    }
+};
+
+template <class SRC>
+__global__ __launch_bounds__(kBlock) void plane_brick_kernel(SRC src, uint32_t n0, uint32_t n1, uint32_t n2,
+                                                             OccTable occ, const PlaneMap* __restrict__ maps,
+                                                             const PlaneInv* __restrict__ inv,
+                                                             const float* __restrict__ dist, uint32_t n_orient,
+                                                             uint32_t n_dist, uint32_t width, uint32_t height,
+                                                             uint32_t o_chunk, double fix_scale,
+                                                             unsigned long long* __restrict__ acc)
+{
+    __shared__ __attribute__((aligned(16))) float2 box[kPBTexels];
+    const uint32_t nb[3] = { (n0 + kPlaneBrick) / kPlaneBrick, (n1 + kPlaneBrick) / kPlaneBrick,
+                             (n2 + kPlaneBrick) / kPlaneBrick }; // bricks over cells 0 .. n
+    const uint32_t nbricks = nb[0] * nb[1] * nb[2];
+    const uint32_t brick = blockIdx.x % nbricks, ob = (blockIdx.x / nbricks) * o_chunk;
+    const uint32_t oe = min(ob + o_chunk, n_orient);
+    const uint32_t bx = brick % nb[0], by = (brick / nb[0]) % nb[1], bz = brick / (nb[0] * nb[1]);
+    const uint32_t nn[3] = { n0, n1, n2 };
+    const uint32_t c0[3] = { bx * kPlaneBrick, by * kPlaneBrick, bz * kPlaneBrick };
+    uint32_t nc[3]; // cells of this brick per axis
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nc[k] = min(uint32_t(kPlaneBrick), nn[k] + 1u - c0[k]);
+    if (occ.t) { // the brick's samples read voxels c0 .. c0 + nc
+        uint32_t lo[3], hi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t last = occ.n[k] - 1u;
+            lo[k] = min(c0[k] >> kOccShift, last);
+            hi[k] = min((c0[k] + nc[k]) >> kOccShift, last) + 1u;
+        }
+        const uint32_t nx = occ.n[0] + 1u, nxy = nx * (occ.n[1] + 1u);
+        typedef __attribute__((address_space(4))) const uint32_t occ_word;
+        const occ_word* t = (const occ_word*)(occ.t);
+        auto at = [&](uint32_t i, uint32_t j, uint32_t k) { return t[k * nxy + j * nx + i]; };
+        const uint32_t cnt = at(hi[0], hi[1], hi[2]) - at(lo[0], hi[1], hi[2]) - at(hi[0], lo[1], hi[2]) -
+                             at(hi[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) + at(lo[0], hi[1], lo[2]) +
+                             at(hi[0], lo[1], lo[2]) - at(lo[0], lo[1], lo[2]);
+        if (cnt == 0) return; // uniform: every sample of the brick is exactly 0
+    }
+    // stage the brick's texels (texels past the brick's cells read as 0 and are never sampled)
+    for (uint32_t t = threadIdx.x; t < uint32_t(kPBTexels); t += kBlock) {
+        const uint32_t lx = t % kPBX, ly = (t / kPBX) % kPBY, lz = t / (kPBX * kPBY);
+        const bool in = lx <= nc[0] && ly <= nc[1] && lz < nc[2];
+        box[t] = in ? src(c0[0] + lx, c0[1] + ly, c0[2] + lz) : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float qlo[3] = { float(c0[0]), float(c0[1]), float(c0[2]) };
+    const float qhi[3] = { float(c0[0] + nc[0]), float(c0[1] + nc[1]), float(c0[2] + nc[2]) };
+    const uint32_t base8 = (c0[2] * uint32_t(kPBX * kPBY) + c0[1] * uint32_t(kPBX) + c0[0]) * 8u;
+    for (uint32_t o = ob + wave; o < oe; o += kBlock / 64) {
+        typedef __attribute__((address_space(4))) const PlaneMap map_const;
+        typedef __attribute__((address_space(4))) const PlaneInv inv_const;
+        const map_const& m = *(const map_const*)(maps + o);
+        const inv_const& mi = *(const inv_const*)(inv + o);
+        float h[3][3], c[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            h[k][0] = m.h[4 * k];
+            h[k][1] = m.h[4 * k + 1];
+            h[k][2] = m.h[4 * k + 3] + 0.5f;
+            c[k] = m.c[k];
+        }
+        // (X, Y, D) = template pixel and distance of a position, affine in q: at the brick's
+        // lowest corner and per unit of each axis' extent
+        float v0[3], vk[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            v0[r] = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v0[r] = fmaf(mi.m[r][k], qlo[k] - h[k][2], v0[r]);
+                vk[r][k] = mi.m[r][k] * float(nc[k]);
+            }
+        }
+        float dmin = v0[2], dmax = v0[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dmin += fminf(vk[2][k], 0.0f);
+            dmax += fmaxf(vk[2][k], 0.0f);
+        }
+        const float dm = 1e-3f + 1e-5f * (fabsf(dmin) + fabsf(dmax));
+        for (uint32_t jb = 0; jb < n_dist; jb += 64) {
+            const uint32_t j = jb + lane;
+            const float d = j < n_dist ? dist[j] : 0.0f;
+            bool ok = j < n_dist && d >= dmin - dm && d <= dmax + dm;
+            // bounding rectangle of the plane's polygon in the brick: the plane cuts the brick's
+            // edges where D - d changes sign
+            float xl = FLT_MAX, xh = -FLT_MAX, yl = FLT_MAX, yh = -FLT_MAX;
+            if (ok) {
+                float cx[8], cy[8], cf[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    float X = v0[0], Y = v0[1], D = v0[2];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        if ((q >> k) & 1) {
+                            X += vk[0][k];
+                            Y += vk[1][k];
+                            D += vk[2][k];
+                        }
+                    cx[q] = X;
+                    cy[q] = Y;
+                    cf[q] = D - d;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        if ((q >> k) & 1) continue;
+                        const int r = q | (1 << k);
+                        const float fa = cf[q], fb = cf[r];
+                        if ((fa <= 0.0f) != (fb <= 0.0f) || fa == 0.0f) {
+                            const float den = fa - fb;
+                            const float t = den != 0.0f ? fminf(fmaxf(fa / den, 0.0f), 1.0f) : 0.0f;
+                            const float X = fmaf(t, cx[r] - cx[q], cx[q]), Y = fmaf(t, cy[r] - cy[q], cy[q]);
+                            xl = fminf(xl, X);
+                            xh = fmaxf(xh, X);
+                            yl = fminf(yl, Y);
+                            yh = fmaxf(yh, Y);
+                        }
+                    }
+            }
+            const float xm = 0.05f + 1e-5f * fmaxf(fabsf(xl), fabsf(xh)), ym = 0.05f + 1e-5f * fmaxf(fabsf(yl), fabsf(yh));
+            const int ixl = int(fmaxf(ceilf(xl - xm), 0.0f));
+            const int ixh = int(fminf(floorf(xh + xm), float(width - 1u)));
+            const int iyl = int(fmaxf(ceilf(yl - ym), 0.0f));
+            const int iyh = int(fminf(floorf(yh + ym), float(height - 1u)));
+            ok = ok && xl <= xh && ixl <= ixh && iyl <= iyh;
+            uint64_t todo = __ballot(ok);
+            while (todo) {
+                const int L = __builtin_ctzll(todo);
+                todo &= todo - 1;
+                const int rxl = __builtin_amdgcn_readlane(ixl, L), rxh = __builtin_amdgcn_readlane(ixh, L);
+                const int ryl = __builtin_amdgcn_readlane(iyl, L), ryh = __builtin_amdgcn_readlane(iyh, L);
+                const float dj = __uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(d), L));
+                float sum = 0.0f;
+                // the rectangle's pixels in row-major order, 64 a pass: lane pixel (px, py) relative
+                // to the rectangle, advanced by 64 pixels with a carry
+                const int w = rxh - rxl + 1, hgt = ryh - ryl + 1;
+                const float rw = 1.0f / float(w);
+                int py = int(float(lane) * rw), px = int(lane) - py * w;
+                if (px < 0) { --py; px += w; } else if (px >= w) { ++py; px -= w; }
+                int dy = int(64.0f * rw), dx = 64 - dy * w;
+                if (dx < 0) { --dy; dx += w; } else if (dx >= w) { ++dy; dx -= w; }
+                for (int base = 0; base < w * hgt; base += 64) {
+                    const int x = rxl + px, y = ryl + py;
+                    const float fx = float(x), fy = float(y);
+                    // the sample position as plane_kernel computes it
+                    const float qx = fmaf(dj, c[0], fmaf(h[0][0], fx, fmaf(h[0][1], fy, h[0][2])));
+                    const float qy = fmaf(dj, c[1], fmaf(h[1][0], fx, fmaf(h[1][1], fy, h[1][2])));
+                    const float qz = fmaf(dj, c[2], fmaf(h[2][0], fx, fmaf(h[2][1], fy, h[2][2])));
+                    const bool mine = py < hgt && qx >= qlo[0] && qx < qhi[0] && qy >= qlo[1] && qy < qhi[1] &&
+                                      qz >= qlo[2] && qz < qhi[2];
+                    if (mine) {
+                        const uint32_t ofs = __umul24(uint32_t(qz), uint32_t(kPBX * kPBY * 8)) +
+                                             (__umul24(uint32_t(qy), uint32_t(kPBX * 8)) + ((uint32_t(qx) << 3) - base8));
+                        const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(box) + ofs);
+                        const float2* q = p + kPBX;
+                        sum += triDeltaZ(p[0], p[1], q[0], q[1], __builtin_amdgcn_fractf(qx),
+                                         __builtin_amdgcn_fractf(qy), __builtin_amdgcn_fractf(qz));
+                    }
+                    px += dx;
+                    py += dy;
+                    if (px >= w) { px -= w; ++py; }
+                }
+                sum = waveSum(sum);
+                if (lane == 0 && sum != 0.0f)
+                    atomicAdd(acc + size_t(o) * n_dist + jb + uint32_t(L),
+                              (unsigned long long)__double2ll_rn(double(sum) * fix_scale));
+            }
+        }
+    }
+}
+
+// out[d * out_stride + o] = scale * total(o, d) of the fixed-point totals.
+__global__ __launch_bounds__(kBlock) void plane_fix_kernel(const unsigned long long* __restrict__ acc, uint32_t n_orient,
+                                                           uint32_t n_dist, double inv_fix, float scale,
+                                                           float* __restrict__ out, uint32_t out_stride)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_orient * n_dist) return;
+    const uint32_t o = i % n_orient, d = i / n_orient;
+    const float total = float(double((long long)acc[size_t(o) * n_dist + d]) * inv_fix);
+    out[size_t(d) * out_stride + o] = total * scale;
+}
+
+// Largest |value| of a brick copy's voxels (the v of each 8-byte texel of every 16-byte element
+// half), as the bits of a non-negative float into *d_max (atomicMax on the bits orders them as the
+// values); NaN and infinity give bits at or above 0x7f800000.
+__global__ __launch_bounds__(kBlock) void brick_maxabs_kernel(const float2* __restrict__ t, size_t n, uint32_t* d_max)
+{
+    uint32_t m = 0;
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock)
+        m = max(m, __float_as_uint(t[i].x) & 0x7fffffffu);
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), s, 64)));
+    if ((threadIdx.x & 63) == 0) atomicMax(d_max, m);
+}
+
 #define CTLK_DISPATCH(KERNEL, vol, grid, ...)                                                       \
     do {                                                                                           \
         const bool il_ = (vol).layout == Layout::Interleaved;                                      \
@@ -523,6 +773,58 @@ hipError_t launchPlaneIntegrals(const VolumeView& vol, const float* d_brick, Bri
     const uint32_t rb = uint32_t((uint64_t(job.n_orient) * job.n_dist + kBlock - 1) / kBlock);
     plane_reduce_kernel<<<rb, kBlock, 0, s>>>(d_partial, job.n_orient, job.n_dist, job.splits, scale, d_out,
                                               out_stride);
+    return hipGetLastError();
+}
+
+uint32_t planeBricks(const uint32_t n[3])
+{
+    return ((n[0] + kPlaneBrick) / kPlaneBrick) * ((n[1] + kPlaneBrick) / kPlaneBrick) *
+           ((n[2] + kPlaneBrick) / kPlaneBrick);
+}
+
+hipError_t launchBrickMaxAbs(const float* d_brick, size_t bytes, uint32_t* d_max, hipStream_t s)
+{
+    const size_t n = bytes / 8;
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    if (n == 0) return hipSuccess;
+    brick_maxabs_kernel<<<uint32_t(blocks), kBlock, 0, s>>>(reinterpret_cast<const float2*>(d_brick), n, d_max);
+    return hipGetLastError();
+}
+
+hipError_t launchPlaneBricks(const VolumeView& vol, const float* d_brick, BrickKind brick_kind, const PlaneMap* d_maps,
+                             const PlaneInv* d_inv, const float* d_dist, uint32_t n_orient, uint32_t n_dist,
+                             uint32_t width, uint32_t height, uint32_t o_chunk, int fix_shift,
+                             unsigned long long* d_acc, float scale, float* d_out, uint32_t out_stride,
+                             hipStream_t s)
+{
+    if (n_orient == 0 || n_dist == 0) return hipSuccess;
+    if (!d_brick || o_chunk == 0 || width == 0 || height == 0 || fix_shift < 0 || fix_shift > 62)
+        return hipErrorInvalidValue;
+    uint32_t nb[3];
+    size_t bytes = 0;
+    BrickKind k;
+    if (!brickShape(vol.n, brick_kind, &k, nb, &bytes) || k != brick_kind) return hipErrorInvalidValue;
+    const uint64_t count = uint64_t(n_orient) * n_dist;
+    const uint64_t blocks = uint64_t(planeBricks(vol.n)) * ((n_orient + o_chunk - 1) / o_chunk);
+    if (blocks > 0x7fffffffull || count > 0xffffffffull - kBlock) return hipErrorInvalidConfiguration;
+    hipError_t e = launchZero(reinterpret_cast<float*>(d_acc), size_t(count) * 2, s);
+    if (e != hipSuccess) return e;
+    const OccTable oc{ vol.occ, { vol.occ_n[0], vol.occ_n[1], vol.occ_n[2] } };
+    const double fix = std::ldexp(1.0, fix_shift);
+    if (k == BrickKind::Texel8) {
+        const Texel8Src f{ d_brick, uint32_t(bytes), nb[0] * 128u, nb[0] * nb[1] * 128u };
+        plane_brick_kernel<Texel8Src><<<uint32_t(blocks), kBlock, 0, s>>>(
+            f, vol.n[0], vol.n[1], vol.n[2], oc, d_maps, d_inv, d_dist, n_orient, n_dist, width, height, o_chunk, fix,
+            d_acc);
+    } else {
+        const Pair16Src f{ d_brick, uint32_t(bytes), nb[0] * 128u, nb[0] * nb[1] * 128u };
+        plane_brick_kernel<Pair16Src><<<uint32_t(blocks), kBlock, 0, s>>>(
+            f, vol.n[0], vol.n[1], vol.n[2], oc, d_maps, d_inv, d_dist, n_orient, n_dist, width, height, o_chunk, fix,
+            d_acc);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t rb = uint32_t((count + kBlock - 1) / kBlock);
+    plane_fix_kernel<<<rb, kBlock, 0, s>>>(d_acc, n_orient, n_dist, 1.0 / fix, scale, d_out, out_stride);
     return hipGetLastError();
 }
 

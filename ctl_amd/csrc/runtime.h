@@ -32,6 +32,8 @@ struct ctlproj_volume {
     // FP32 brick copy for the plane integrals and its kind, made on first use (guarded by ctx->mu)
     mutable float* d_brick = nullptr;
     mutable ctlk::BrickKind brick_kind = ctlk::BrickKind::Auto;
+    // largest |voxel| of the brick copy (negative: not measured yet; NaN: not finite)
+    mutable float brick_maxabs = -1.0f;
 };
 
 struct ctlproj_projector {

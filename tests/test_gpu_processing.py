@@ -166,11 +166,12 @@ def test_plane_integrals_against_oracle(storage, layout):
 
 @pytest.mark.parametrize("storage", STORAGES)
 def test_plane_integrals_brick_and_texel_reads_agree(storage, monkeypatch):
-    """The plane kernel reads an FP32 brick copy of the volume by default (8-byte texel bricks for
-    a volume this small; CTLPROJ_PLANE_READS=pairs asks for the 16-byte pair bricks that larger
-    volumes get) and the resident texels with CTLPROJ_PLANE_READS=texels: the same sample positions
-    and interpolation order, so FP32 and BF16 agree bit for bit; FP16 interpolates its halves in
-    another FMA order (within 1e-6)."""
+    """By default the integrals are summed brick by brick from LDS, staged from the volume's FP32
+    brick copy (8-byte texel bricks for a volume this small; CTLPROJ_PLANE_READS=pairs asks for the
+    16-byte pair bricks that larger volumes get): both copies hold the same texels, so the sums
+    agree bit for bit. CTLPROJ_PLANE_READS=texels (and CTLPROJ_PLANE_KERNEL=tiles over the brick
+    copy) run the tile kernel, which gathers the same samples through the caches and adds them in
+    another order: within 1e-6 of the largest integral."""
     vol = random_volume(31, shape=(26, 22, 30))
     rng = np.random.default_rng(32)
     normals = random_normals(rng, 7)
@@ -180,20 +181,23 @@ def test_plane_integrals_brick_and_texel_reads_agree(storage, monkeypatch):
     pairs = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
     monkeypatch.setenv("CTLPROJ_PLANE_READS", "texels")
     texels = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
-    for got in (bricks, pairs):
-        if storage == "f16":
-            assert np.abs(got - texels).max() <= 1e-6 * np.abs(texels).max()
-        else:
-            assert np.array_equal(got, texels)
+    monkeypatch.delenv("CTLPROJ_PLANE_READS")
+    monkeypatch.setenv("CTLPROJ_PLANE_KERNEL", "tiles")
+    tiles = RadonTransform3D(vol, storage).planeIntegrals(normals, dist)
+    assert np.array_equal(bricks, pairs)
+    for got in (texels, tiles):
+        assert np.abs(got - bricks).max() <= 1e-6 * np.abs(bricks).max()
     side, reso = 48, 0.8
     r32 = oracle.plane_integrals(rounded(vol.data, storage), vol.voxel_size, vol.offset, normals, dist, side, side,
                                  reso)
-    assert np.abs(bricks - r32).max() <= 2e-5 * np.abs(r32).max()
+    for got in (bricks, tiles):
+        assert np.abs(got - r32).max() <= 2e-5 * np.abs(r32).max()
 
 
 def test_plane_integrals_brick_kind_switch_on_one_volume(monkeypatch):
     """One resident volume asked for pair bricks after its texel bricks were made (the copy is
-    replaced) and back: every result equals the texel reads bit for bit."""
+    replaced) and back: every brick-kernel result is the same bit for bit, the tile kernel's within
+    1e-6."""
     vol = random_volume(33, shape=(20, 24, 18))
     rng = np.random.default_rng(34)
     normals = random_normals(rng, 5)
@@ -206,18 +210,36 @@ def test_plane_integrals_brick_kind_switch_on_one_volume(monkeypatch):
     third = rt.planeIntegrals(normals, dist)
     monkeypatch.setenv("CTLPROJ_PLANE_READS", "texels")
     texels = rt.planeIntegrals(normals, dist)
-    for got in (first, second, third):
-        assert np.array_equal(got, texels)
+    for got in (second, third):
+        assert np.array_equal(got, first)
+    assert np.abs(texels - first).max() <= 1e-6 * np.abs(first).max()
+
+
+def test_plane_integrals_non_finite_volume_uses_tile_kernel(monkeypatch):
+    """A volume with an infinite voxel has no fixed-point scale for the brick kernel's totals: the
+    tile kernel sums its planes, and the planes through the voxel are not finite, as in the
+    reference's float sums."""
+    data = random_volume(35, shape=(14, 16, 12)).data.copy()
+    data[6, 8, 7] = np.inf
+    vol = VoxelVolume(data, (1, 1, 1), (0, 0, 0))
+    normals = np.eye(3)
+    dist = np.array([-3.0, 0.0, 3.0], np.float32)
+    got = RadonTransform3D(vol).planeIntegrals(normals, dist)
+    monkeypatch.setenv("CTLPROJ_PLANE_KERNEL", "tiles")
+    tiles = RadonTransform3D(vol).planeIntegrals(normals, dist)
+    assert np.array_equal(got, tiles, equal_nan=True)
+    assert not np.isfinite(got).all() and np.isfinite(got).any()
 
 
 @pytest.mark.parametrize("storage", STORAGES)
-@pytest.mark.parametrize("reads", ["bricks", "pairs", "texels"])
+@pytest.mark.parametrize("reads", ["bricks", "pairs", "texels", "tiles"])
 def test_plane_integrals_skip_empty_space_exactly(storage, reads, monkeypatch):
-    """The plane kernel skips the distances of a tile whose samples read only empty 4^3-voxel
-    bricks of the volume's occupancy table. Those samples are exactly 0: on a phantom with empty
-    space, and on a volume of a few non-zero voxels (at the grid's corners too), the integrals
-    equal the ones taken over a volume without a table (CTLPROJ_EMPTY_SKIP=0) bit for bit, and the
-    oracle."""
+    """Both plane-integral kernels skip empty space with the volume's occupancy table: the brick
+    kernel its bricks that hold no non-zero voxel, the tile kernel ("texels", "tiles") the
+    distances of a tile whose samples read only empty 4^3-voxel bricks. Those samples are exactly
+    0: on a phantom with empty space, and on a volume of a few non-zero voxels (at the grid's
+    corners too), the integrals equal the ones taken over a volume without a table
+    (CTLPROJ_EMPTY_SKIP=0) bit for bit, and the oracle."""
     from ctl_amd import workloads as W
     rng = np.random.default_rng(41)
     sparse = np.zeros((27, 31, 23), np.float32)
@@ -226,7 +248,9 @@ def test_plane_integrals_skip_empty_space_exactly(storage, reads, monkeypatch):
         sparse[x, y, z] = rng.uniform(0.5, 1.0)
     phantom = W.small(views=1).volume[::2, ::2, ::2].copy()
     assert (phantom == 0).mean() > 0.3
-    if reads != "bricks":
+    if reads == "tiles":
+        monkeypatch.setenv("CTLPROJ_PLANE_KERNEL", "tiles")
+    elif reads != "bricks":
         monkeypatch.setenv("CTLPROJ_PLANE_READS", reads)
     normals = np.vstack([np.eye(3), random_normals(rng, 9)])
     for data, voxel, offset in ((sparse, (0.9, 1.1, 0.8), (1.5, -2.0, 0.7)), (phantom, (2.0, 2.0, 2.0), (0, 0, 0))):
@@ -299,8 +323,11 @@ def test_plane_integrals_device_output():
     assert np.array_equal(out.cpu().numpy().reshape(11, 4), rt.planeIntegrals(normals, dist))
 
 
-def test_many_orientations_split_over_launches():
-    # enough orientations x distances that the host splits them over several launches
+@pytest.mark.parametrize("kernel", ["bricks", "tiles"])
+def test_many_orientations_split_over_launches(kernel, monkeypatch):
+    # enough orientations x distances that the tile kernel's host splits them over several launches
+    if kernel == "tiles":
+        monkeypatch.setenv("CTLPROJ_PLANE_KERNEL", "tiles")
     vol = random_volume(15, shape=(12, 12, 12), voxel=(1, 1, 1), offset=(0, 0, 0))
     rt = RadonTransform3D(vol)
     rng = np.random.default_rng(16)

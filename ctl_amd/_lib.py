@@ -92,7 +92,10 @@ class ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("submitted", "completed", "failed", "rejected", "views",
                                           "launches")] + \
                [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "p50_single_ms",
-                                          "p99_single_ms", "p50_multi_ms", "p99_multi_ms", "span_s")]
+                                          "p99_single_ms", "p50_multi_ms", "p99_multi_ms", "span_s")] + \
+               [(n, C.c_uint32) for n in ("slots", "cu_masked_slots", "max_scan_launches_in_flight",
+                                          "reserved")] + \
+               [("drr_launches_beside_scan", C.c_uint64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_int, C.c_void_p)

@@ -121,6 +121,8 @@ struct ctlproj_server {
 
     // statistics since the last reset
     uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, views = 0, launches = 0;
+    uint32_t cu_masked_slots = 0, max_scans_in_flight = 0;
+    uint64_t drr_beside_scan = 0;
     std::vector<double> lat_single, lat_multi;
     Clock::time_point first_submit{}, last_done{};
     bool any_submit = false;
@@ -336,6 +338,11 @@ void launchLocked(ctlproj_server* S, Slot& slot, bool drr_only)
     slot.busy = true;
     slot.drr = drr;
     ++S->launches;
+    if (!drr) {
+        uint32_t scans = 0;
+        for (const Slot& sl : S->slots) scans += (sl.busy && !sl.drr) ? 1u : 0u;
+        S->max_scans_in_flight = std::max(S->max_scans_in_flight, scans);
+    }
 }
 
 void schedulerMain(ctlproj_server* S)
@@ -380,6 +387,12 @@ void completerMain(ctlproj_server* S)
             if (q == hipErrorNotReady) continue;
             progress = true;
             slot.busy = false;
+            if (q == hipSuccess && slot.drr)
+                for (const Slot& other : S->slots)
+                    if (other.busy && !other.drr && hipEventQuery(other.done) == hipErrorNotReady) {
+                        ++S->drr_beside_scan;
+                        break;
+                    }
             if (q != hipSuccess) {
                 failChunksLocked(S, slot.chunks, hipFail(q, "serving kernel"));
                 continue;
@@ -457,10 +470,14 @@ ctlproj_status ctlproj_server_create(ctlproj_context* ctx, size_t hbm_budget, ui
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->ctx->device);
     const std::vector<uint32_t> all_cus((uint32_t(cus) + 31) / 32, ~0u);
     for (Slot& sl : S->slots) {
-        if (e == hipSuccess && hipExtStreamCreateWithCUMask(&sl.stream, uint32_t(all_cus.size()), all_cus.data()) != hipSuccess) {
-            (void)hipGetLastError();
-            sl.stream = nullptr;
-            e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) {
+            if (hipExtStreamCreateWithCUMask(&sl.stream, uint32_t(all_cus.size()), all_cus.data()) == hipSuccess) {
+                ++S->cu_masked_slots;
+            } else {
+                (void)hipGetLastError();
+                sl.stream = nullptr;
+                e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+            }
         }
         if (e == hipSuccess) e = hipMalloc(&sl.d_items, kSlotItems * sizeof(ctlk::BatchItem));
         if (e == hipSuccess) e = hipHostMalloc(&sl.h_items, kSlotItems * sizeof(ctlk::BatchItem), hipHostMallocDefault);
@@ -614,6 +631,10 @@ ctlproj_status ctlproj_server_get_stats(ctlproj_server* S, ctlproj_server_stats*
     st.p50_multi_ms = percentile(S->lat_multi, 0.50);
     st.p99_multi_ms = percentile(S->lat_multi, 0.99);
     st.span_s = (S->any_submit && S->completed) ? std::chrono::duration<double>(S->last_done - S->first_submit).count() : 0.0;
+    st.slots = kSlots;
+    st.cu_masked_slots = S->cu_masked_slots;
+    st.max_scan_launches_in_flight = S->max_scans_in_flight;
+    st.drr_launches_beside_scan = S->drr_beside_scan;
     *out = st;
     return CTLPROJ_OK;
 }
@@ -623,6 +644,8 @@ ctlproj_status ctlproj_server_reset_stats(ctlproj_server* S)
     if (!S) return fail(CTLPROJ_ERR_INVALID_ARGUMENT, "server is NULL");
     std::lock_guard<std::mutex> lk(S->mu);
     S->submitted = S->completed = S->failed = S->rejected = S->views = S->launches = 0;
+    S->max_scans_in_flight = 0;
+    S->drr_beside_scan = 0;
     S->lat_single.clear();
     S->lat_multi.clear();
     S->any_submit = false;

@@ -283,6 +283,12 @@ typedef struct ctlproj_server_stats {
     double p50_single_ms, p99_single_ms;             /* one-view requests */
     double p50_multi_ms, p99_multi_ms;               /* requests of more than one view */
     double span_s;       /* first submit to last completion */
+    /* launch slots: how many slot streams carry a CU mask of the whole device (each then has a
+     * hardware queue of its own; 0 if the runtime refused the mask and plain streams were made),
+     * the most scan launches (more than one view) in flight at once (at most slots - 1), and the
+     * single-view launches that completed while a scan launch issued before them was still running */
+    uint32_t slots, cu_masked_slots, max_scan_launches_in_flight, reserved;
+    uint64_t drr_launches_beside_scan;
 } ctlproj_server_stats;
 
 /* hbm_budget: bytes of the arena (outputs + geometry of admitted requests).

@@ -126,8 +126,18 @@ def test_single_view_requests_pass_a_running_scan(phantoms):
     dids = [srv.submit(vol, pm, 128, 128, o, st) for pm, o in zip(drr_pms, drr_outs)]
     drr_lat = [srv.wait(i) for i in dids]
     scan_lat = srv.wait(sid)
-    print(f"scan {scan_lat:.2f} ms, DRRs {', '.join(f'{t:.2f}' for t in drr_lat)} ms")
+    st = srv.stats()
+    print(f"scan {scan_lat:.2f} ms, DRRs {', '.join(f'{t:.2f}' for t in drr_lat)} ms; {st['launches']} launches, "
+          f"{st['cu_masked_slots']}/{st['slots']} CU-masked slot streams, at most "
+          f"{st['max_scan_launches_in_flight']} scan launches in flight, {st['drr_launches_beside_scan']} DRR "
+          f"launches done beside a running scan launch")
     assert max(drr_lat) < scan_lat
+    # the launch slots: every slot stream has a hardware queue of its own (a CU-masked stream; a
+    # silent fall back to plain streams would queue DRRs behind scans), scans never take the last
+    # slot, and DRR launches complete while a scan launch of the same run is still running
+    assert st["slots"] == 3 and st["cu_masked_slots"] == 3
+    assert 1 <= st["max_scan_launches_in_flight"] <= st["slots"] - 1
+    assert st["drr_launches_beside_scan"] >= 1
     assert np.array_equal(scan_out, scan_alone)
     for a, o in zip(drr_alone, drr_outs):
         assert np.array_equal(a, o)

@@ -69,3 +69,32 @@ def test_oracle_reproduces_reference_spectral_projections(density_projs, name):
     print(f"{name}: mean diff {mean:.2e}, variance {var:.2e}, max |diff| {worst:.2e}")
     assert abs(mean) < 0.01 and var < 0.01  # projectortest.cpp:96-144
     assert worst < 2e-4
+
+
+@pytest.mark.parametrize("mistake", ["bin_shift", "no_energy_weight", "step", "ball_centre"])
+def test_planted_mistakes_fail_the_fixture_bar(density_projs, mistake):
+    """Each plausible mistake in the chain moves some pixel of spectral_lin_simple.den by more than
+    the 2e-4 bar: attenuation coefficients taken one energy bin off, photon counts instead of
+    energy-weighted intensities, a ray step of 0.33 instead of 0.3 voxel, a ball centred at n / 2
+    instead of (n - 1) / 2."""
+    d = S.inputs()
+    mu = np.array(d["mass_attenuation_cm2_g"]["water"], np.float32)[:, None] * np.float32(0.1)
+    inten = np.array(d["intensities"])
+    proj = density_projs[0]
+    if mistake == "bin_shift":
+        mu = np.roll(mu, 1, axis=0)
+    elif mistake == "no_energy_weight":
+        inten = inten / np.array(d["energies_keV"])[:, None]
+    elif mistake == "step":
+        proj = oracle.project(S.ball(40.0, 0.5, 1.0), (0.5, 0.5, 0.5), (0.0, 0.0, 0.0), S.geometry(), 50, 50,
+                              0.33)[0][:, 0]
+    elif mistake == "ball_centre":
+        n = 160
+        i = np.arange(n, dtype=np.float32) - np.float32(n / 2)
+        d2 = i[:, None, None] ** 2 + i[None, :, None] ** 2 + i[None, None, :] ** 2
+        ball = np.where(d2 > np.float32(80.0 ** 2), np.float32(0.0), np.float32(1.0)).astype(np.float32)
+        proj = oracle.project(ball, (0.5, 0.5, 0.5), (0.0, 0.0, 0.0), S.geometry(), 50, 50, 0.3)[0][:, 0]
+    got = oracle.spectral_extinction(proj[None], mu, inten)
+    worst = S.stats(got, S.expected("lin_simple"))[2]
+    print(f"{mistake}: max |diff| {worst:.2e}")
+    assert worst > 2e-4

@@ -424,6 +424,21 @@ struct Pair16Src {
     }
 };
 
+// Sum over the wave's 64 lanes (all active), the same value returned to every lane: DPP within
+// each 16-lane row, then the four rows.
+__device__ __forceinline__ float waveSumDpp(float v)
+{
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x124, 0xF, 0xF, false)); // row_ror:4
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false)); // row_ror:8
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
+}
+
 template <class SRC>
 __global__ __launch_bounds__(kBlock) void plane_brick_kernel(SRC src, uint32_t n0, uint32_t n1, uint32_t n2,
                                                              OccTable occ, const PlaneMap* __restrict__ maps,
@@ -434,6 +449,8 @@ __global__ __launch_bounds__(kBlock) void plane_brick_kernel(SRC src, uint32_t n
                                                              unsigned long long* __restrict__ acc)
 {
     __shared__ __attribute__((aligned(16))) float2 box[kPBTexels];
+    __shared__ uint32_t s_j[kBlock / 64][64]; // each wave's list of distances to cut the brick
+    __shared__ float s_d[kBlock / 64][64];
     const uint32_t nb[3] = { (n0 + kPlaneBrick) / kPlaneBrick, (n1 + kPlaneBrick) / kPlaneBrick,
                              (n2 + kPlaneBrick) / kPlaneBrick }; // bricks over cells 0 .. n
     const uint32_t nbricks = nb[0] * nb[1] * nb[2];
@@ -505,10 +522,14 @@ __global__ __launch_bounds__(kBlock) void plane_brick_kernel(SRC src, uint32_t n
             dmax += fmaxf(vk[2][k], 0.0f);
         }
         const float dm = 1e-3f + 1e-5f * (fabsf(dmin) + fabsf(dmax));
-        for (uint32_t jb = 0; jb < n_dist; jb += 64) {
-            const uint32_t j = jb + lane;
-            const float d = j < n_dist ? dist[j] : 0.0f;
-            bool ok = j < n_dist && d >= dmin - dm && d <= dmax + dm;
+        // the distances whose plane can cut the brick, gathered into this wave's list (64 at a
+        // time) so that one pass of the wave's lanes computes their polygons
+        uint32_t listed = 0;
+        auto flush = [&]() {
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t j = lane < listed ? s_j[wave][lane] : 0u;
+            const float d = s_d[wave][lane];
+            bool ok = lane < listed;
             // bounding rectangle of the plane's polygon in the brick: the plane cuts the brick's
             // edges where D - d changes sign
             float xl = FLT_MAX, xh = -FLT_MAX, yl = FLT_MAX, yh = -FLT_MAX;
@@ -589,12 +610,30 @@ __global__ __launch_bounds__(kBlock) void plane_brick_kernel(SRC src, uint32_t n
                     py += dy;
                     if (px >= w) { px -= w; ++py; }
                 }
-                sum = waveSum(sum);
+                sum = waveSumDpp(sum);
                 if (lane == 0 && sum != 0.0f)
-                    atomicAdd(acc + size_t(o) * n_dist + jb + uint32_t(L),
+                    atomicAdd(acc + size_t(o) * n_dist + uint32_t(__builtin_amdgcn_readlane(int(j), L)),
                               (unsigned long long)__double2ll_rn(double(sum) * fix_scale));
             }
+            __builtin_amdgcn_wave_barrier();
+            listed = 0;
+        };
+        for (uint32_t jb = 0; jb < n_dist; jb += 64) {
+            const uint32_t j = jb + lane;
+            const float d = j < n_dist ? dist[j] : 0.0f;
+            const bool in = j < n_dist && d >= dmin - dm && d <= dmax + dm;
+            const uint64_t m = __ballot(in);
+            const uint32_t cnt = uint32_t(__popcll(m));
+            if (cnt == 0) continue; // uniform
+            if (listed + cnt > 64) flush();
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+            if (in) {
+                s_j[wave][listed + rank] = j;
+                s_d[wave][listed + rank] = d;
+            }
+            listed += cnt;
         }
+        if (listed) flush();
     }
 }
 

@@ -328,7 +328,7 @@ def bench_processing(args, torch, w, ctl, oracle):
                   "frac": gather / LDS_PEAK_GBPS, "frac_of_l1_peak": gather / L1_PEAK_GBPS,
                   "note": "achieved = samples inside the support x 8 taps x 4 B per wall second "
                           "(samples of empty bricks are skipped with the same sums); the brick "
-                          "kernel reads its taps from bricks of 16^3 cells staged in LDS (two "
+                          "kernel reads its taps from bricks of 14^3 cells staged in LDS (two "
                           "ds_read2_b64 per sample) and is bound by instruction issue and LDS bank "
                           "conflicts (counters in profiles/plane_brick_kernel_f32_256cube_1024orient_128dist.md)"})}
     log(f"radon3d: {json.dumps(res['radon3d'])}")

@@ -398,9 +398,9 @@ __global__ __launch_bounds__(kBlock) void plane_reduce_kernel(const float* __res
 // integer (2^fix_shift units per value), so the totals do not depend on the order in which
 // workgroups finish; plane_fix_kernel converts them back. The host picks fix_shift from the
 // volume's largest |value| so that no total can overflow.
-constexpr int kPlaneBrick = 16;
+constexpr int kPlaneBrick = 14;
 constexpr int kPBX = kPlaneBrick + 1, kPBY = kPlaneBrick + 1, kPBZ = kPlaneBrick;
-constexpr int kPBTexels = kPBX * kPBY * kPBZ; // 4624 texels, 36,992 bytes of LDS
+constexpr int kPBTexels = kPBX * kPBY * kPBZ; // 3150 texels, 25,200 bytes of LDS
 
 // Texel (x, y, z) of the padded frame, (v, dz), from a brick copy.
 struct Texel8Src {

@@ -262,7 +262,9 @@ __device__ __forceinline__ bool insideQ(int i, const float s[3], const float c[3
 __device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, const float nhi[3])
 {
     if (!r.hit) { r.i0 = 1; r.i1 = 0; return; }
+#pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
     while (r.i0 <= r.i1 && !insideQ(r.i0, r.s, c, blo, nhi)) ++r.i0;
+#pragma clang loop unroll(disable) vectorize(disable) interleave(disable)
     while (r.i1 >= r.i0 && !insideQ(r.i1, r.s, c, blo, nhi)) --r.i1;
 }
 
@@ -553,25 +555,44 @@ __device__ __forceinline__ uint32_t wavePkMin(uint32_t v)
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Trilinear sample from the LDS box ((v, dz) texels) at byte offset `ofs`, with rows px8 bytes
-// apart; the same arithmetic in the same order as Sampler<F32, Interleaved>.
-__device__ __forceinline__ float triLds(const float2* __restrict__ box, uint32_t ofs, uint32_t px8,
-                                        float wx, float wy, float wz)
+typedef float lds_f2 __attribute__((ext_vector_type(2)));
+typedef uint32_t lds_u2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) const lds_f2 lds_float2;
+typedef __attribute__((address_space(3))) const lds_u2 lds_uint2;
+
+// LDS byte address of a pointer into shared memory
+__device__ __forceinline__ uint32_t ldsAddr(const void* p)
 {
-    const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(box) + ofs);
-    const float2* q = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(p) + px8);
-    const float2 a0 = p[0], a1 = p[1];
-    const float2 b0 = q[0], b1 = q[1];
-    return triDeltaZ(a0, a1, b0, b1, wx, wy, wz);
+    return uint32_t(uintptr_t((__attribute__((address_space(3))) const char*)p));
 }
 
-// FP16/BF16 quads: texels x and x+1 at byte offset `ofs`; the same lerps as QuadHalf::tri.
-template <Storage ST>
-__device__ __forceinline__ float quadLds(const float2* __restrict__ box, uint32_t ofs, float wx, float wy,
-                                         float wz)
+// Trilinear sample from the LDS box ((v, dz) texels) at LDS byte address `at`, with rows px8 bytes
+// apart; the same arithmetic in the same order as Sampler<F32, Interleaved>.
+__device__ __forceinline__ float triLds(uint32_t at, uint32_t px8, float wx, float wy, float wz)
 {
-    const uint2* p = reinterpret_cast<const uint2*>(reinterpret_cast<const char*>(box) + ofs);
-    const uint2 a = p[0], b = p[1];
+    const lds_float2* p = (const lds_float2*)uintptr_t(at);
+    const lds_float2* q = (const lds_float2*)uintptr_t(at + px8);
+    const lds_f2 a0 = p[0], a1 = p[1];
+    const lds_f2 b0 = q[0], b1 = q[1];
+    return triDeltaZ(make_float2(a0.x, a0.y), make_float2(a1.x, a1.y), make_float2(b0.x, b0.y),
+                     make_float2(b1.x, b1.y), wx, wy, wz);
+}
+
+// a * b + c on the low 24 bits of a and b as one v_mad_u32_u24 (b uniform); written out because
+// the compiler otherwise splits two of them into two multiplies and a three-way add
+__device__ __forceinline__ uint32_t mad24(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
+}
+
+// FP16/BF16 quads: texels x and x+1 at LDS byte address `at`; the same lerps as QuadHalf::tri.
+template <Storage ST>
+__device__ __forceinline__ float quadLds(uint32_t at, float wx, float wy, float wz)
+{
+    const lds_uint2* p = (const lds_uint2*)uintptr_t(at);
+    const lds_u2 a = p[0], b = p[1];
     return quadTri<ST>(a.x, a.y, b.x, b.y, wx, wy, wz);
 }
 
@@ -735,7 +756,10 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                 // (cx, cy, cz) of the first by division, then steps of 512 texels with carries
                 int cx, cy, cz, dx, dy, dz;
                 {
-                    const float rpx = 1.0f / float(px), rby = 1.0f / float(nb[1]);
+                    // approximate reciprocals (1 ulp): every quotient below is < 512, so the
+                    // truncated products are off by at most one, which the carries correct
+                    const float rpx = __builtin_amdgcn_rcpf(float(px));
+                    const float rby = __builtin_amdgcn_rcpf(float(nb[1]));
                     const int t0 = int(wave) * 128 + 2 * int(lane);
                     int row = int(float(t0) * rpx);
                     cx = t0 - row * px;
@@ -780,16 +804,18 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
                 // is one shift-add and two 24-bit multiply-adds (mod 2^32; the result is in range)
                 const uint32_t px8 = uint32_t(px) * 8u, pz8 = uint32_t(pz) * 8u;
                 const uint32_t base8 = (uint32_t(o[0]) + uint32_t(o[1]) * uint32_t(px) + uint32_t(o[2]) * uint32_t(pz)) * 8u;
+                // LDS address of texel (ix, iy, iz): ix * 8 + iy * px8 + iz * pz8 + bias (mod 2^32)
+                const uint32_t bias = ldsAddr(box) - base8;
                 sampleRange<UNROLL>(wlo, wa, we, whi, a, e, live, part, [&](float fi) {
                     const float qx = fmaf(fi, ray.s[0], gc[0]);
                     const float qy = fmaf(fi, ray.s[1], gc[1]);
                     const float qz = fmaf(fi, ray.s[2], gc[2]);
-                    const uint32_t ofs = __umul24(uint32_t(int(qz)), pz8) +
-                                         (__umul24(uint32_t(int(qy)), px8) + ((uint32_t(int(qx)) << 3) - base8));
+                    const uint32_t at = mad24(uint32_t(int(qz)), pz8,
+                                              mad24(uint32_t(int(qy)), px8, (uint32_t(int(qx)) << 3) + bias));
                     const float wx = __builtin_amdgcn_fractf(qx), wy = __builtin_amdgcn_fractf(qy);
                     const float wz = __builtin_amdgcn_fractf(qz);
-                    if constexpr (ST == Storage::F32) return triLds(box, ofs, px8, wx, wy, wz);
-                    else return quadLds<ST>(box, ofs, wx, wy, wz);
+                    if constexpr (ST == Storage::F32) return triLds(at, px8, wx, wy, wz);
+                    else return quadLds<ST>(at, wx, wy, wz);
                 }, [&](float fi) {
                     // with unroll 3 (4) a third (a quarter) of the shared body gathers through the
                     // vector L1 instead (the box was just read from there): the LDS pipe and the

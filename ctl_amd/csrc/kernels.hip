@@ -632,6 +632,32 @@ __device__ __forceinline__ bool boxEmpty(const VolumeView& vol, const int o[3], 
     return bricksEmpty(vol, lo, hi);
 }
 
+// Texel box [o, o + nb) per axis of the taps of samples s0..s1 (0 <= s0 <= s1) of rays whose
+// steps lie in [smin, smax] per axis. All rays of a (view, module) start from the same point c, so
+// along each axis the positions fma(i, s, c) lie between fma(i, smin, c) and fma(i, smax, c) (the
+// FMA is monotonic in s for i >= 0, and in i), extremes at i = s0 or s1; positions outside
+// [blo, nhi) are never taken. A tap of position q reads texels floor(q) .. floor(q) + 1 along x,
+// and along y for FP32 (the texels carry z + 1, the FP16/BF16 quads y + 1 too). False if the
+// positions miss [blo, nhi) on some axis.
+template <Storage ST>
+__device__ __forceinline__ bool boxOf(int s0, int s1, const float smin[3], const float smax[3], const float c[3],
+                                      float blo, const float nhi[3], int o[3], int nb[3])
+{
+    bool any = true;
+    const float f0 = float(s0), f1 = float(s1);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float a0 = fmaf(f0, smin[k], c[k]), a1 = fmaf(f1, smin[k], c[k]);
+        const float b0 = fmaf(f0, smax[k], c[k]), b1 = fmaf(f1, smax[k], c[k]);
+        const float ql = fmaxf(fminf(a0, a1), blo), qh = fminf(fmaxf(b0, b1), nhi[k]);
+        const int extra = (k == 0 || (k == 1 && ST == Storage::F32)) ? 1 : 0;
+        any = any && ql <= qh;
+        o[k] = int(ql);
+        nb[k] = min(int(qh), int(nhi[k]) - 1) + extra - o[k] + 1;
+    }
+    return any;
+}
+
 template <Storage ST, int UNROLL, bool LARGE = false>
 __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                                const RayParams& rp, const CastConst& kc,
@@ -668,74 +694,59 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
         const int r_hi = waveReduce<true>(act ? ray.i1 : INT_MIN);
         const int r_alo = waveReduce<true>(act ? ray.i0 : INT_MIN);
         const int r_ahi = waveReduce<false>(act ? ray.i1 : INT_MAX);
-        if (lane == 0) { red[1][wave][0] = r_lo; red[1][wave][1] = r_hi; }
+        // the extremes of the active rays' steps per axis, over the wave and the workgroup (the
+        // texel boxes of the chunks follow from them, boxOf)
+        float smin[3], smax[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            smin[k] = waveMinF(act ? ray.s[k] : FLT_MAX);
+            smax[k] = waveMaxF(act ? ray.s[k] : -FLT_MAX);
+        }
+        if (lane == 0) {
+            red[1][wave][0] = r_lo;
+            red[1][wave][1] = r_hi;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                red[1][wave][2 + k] = __float_as_int(smin[k]);
+                red[1][wave][5 + k] = __float_as_int(smax[k]);
+            }
+        }
         __syncthreads();
         const int blo = min(min(red[1][0][0], red[1][1][0]), min(red[1][2][0], red[1][3][0]));
         const int bhi = max(max(red[1][0][1], red[1][1][1]), max(red[1][2][1], red[1][3][1]));
+        float gmin[3], gmax[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gmin[k] = fminf(fminf(__int_as_float(red[1][0][2 + k]), __int_as_float(red[1][1][2 + k])),
+                            fminf(__int_as_float(red[1][2][2 + k]), __int_as_float(red[1][3][2 + k])));
+            gmax[k] = fmaxf(fmaxf(__int_as_float(red[1][0][5 + k]), __int_as_float(red[1][1][5 + k])),
+                            fmaxf(__int_as_float(red[1][2][5 + k]), __int_as_float(red[1][3][5 + k])));
+        }
         const int K = int(kc.chunk);
         float part[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
-        int parity = 0;
-        for (int c0 = blo; c0 <= bhi; c0 += K, parity ^= 1) {
+        for (int c0 = blo; c0 <= bhi; c0 += K) {
             const int c1 = min(c0 + K - 1, bhi);
             const int a = act ? max(c0, ray.i0) : INT_MAX;
             const int e = act ? min(c1, ray.i1) : INT_MIN;
             const bool live = a <= e;
-            // Texel box of this lane's taps in the chunk, [lo, hi] per axis (texels carry z+1, and
-            // y+1 for the FP16/BF16 quads, so only x and for FP32 y need one more). Reduced over
-            // the wave as packed 16-bit minima (a maximum m as the minimum of 0xffff - m): padded
-            // sizes stay below 65535 when a box is staged (kc.lds_texels > 0). 0xffff marks "no
-            // live lane".
-            uint32_t pk[3];
-            {
-                const int as = live ? a : c0, es = live ? e : c0; // in range for every lane
-                uint32_t lo[3], hi[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float qa = fmaf(float(as), ray.s[k], gc[k]), qe = fmaf(float(es), ray.s[k], gc[k]);
-                    const uint32_t extra = (k == 0 || (k == 1 && ST == Storage::F32)) ? 1u : 0u;
-                    lo[k] = uint32_t(int(fminf(qa, qe))) & 0xffffu;
-                    hi[k] = 0xffffu - ((uint32_t(int(fmaxf(qa, qe))) + extra) & 0xffffu);
-                }
-                pk[0] = live ? lo[0] | lo[1] << 16 : ~0u;
-                pk[1] = live ? hi[0] | hi[1] << 16 : ~0u;
-                pk[2] = live ? lo[2] | hi[2] << 16 : ~0u;
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) pk[k] = wavePkMin(pk[k]);
             // the wave's sample range [wlo, whi] in this chunk and the part [wa, we] all its live
             // lanes share (empty: all masked)
             int wlo = max(c0, r_lo), whi = min(c1, r_hi), wa = max(c0, r_alo), we = min(c1, r_ahi);
             if (wlo > whi) { wlo = INT_MAX; whi = INT_MIN; wa = INT_MIN + 1; we = INT_MIN; }
             else if (wa > we) { wa = whi + 1; we = whi; }
-            if (lane == 0)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) red[parity][wave][k] = int(pk[k]);
-            __syncthreads(); // the box of the previous chunk is no longer read; red[parity] is complete
-            uint32_t bpk[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                bpk[k] = pkMin(pkMin(uint32_t(red[parity][0][k]), uint32_t(red[parity][1][k])),
-                               pkMin(uint32_t(red[parity][2][k]), uint32_t(red[parity][3][k])));
-            const int o[3] = { int(bpk[0] & 0xffffu), int(bpk[0] >> 16), int(bpk[2] & 0xffffu) };
-            const int nb[3] = { int(0xffffu - (bpk[1] & 0xffffu)) - o[0] + 1,
-                                int(0xffffu - (bpk[1] >> 16)) - o[1] + 1,
-                                int(0xffffu - (bpk[2] >> 16)) - o[2] + 1 };
-            // with a box to stage (16-bit coordinates valid), 0xffff marks a chunk in which no lane
-            // of the workgroup samples; without, sampling below finds no live lane
-            if (kc.lds_texels && o[0] == 0xffff) continue; // uniform
-            // a chunk whose box holds only zero voxels adds exactly 0 to every ray (uniform)
-            if (kc.lds_texels && boxEmpty(vol, o, nb)) continue;
-            // so does this wave's part of a chunk whose own box (its lanes' taps) is empty: the wave
-            // then only helps to stage the box for the others (uniform over the wave)
+            // the workgroup's texel box for the chunk (uniform): a chunk whose samples miss the
+            // support, or whose box holds only zero voxels, adds exactly 0 to every ray
+            int o[3] = { 0, 0, 0 }, nb[3] = { 0, 0, 0 };
+            if (kc.lds_texels && (!boxOf<ST>(c0, c1, gmin, gmax, gc, kc.blo, kc.nhi, o, nb) || boxEmpty(vol, o, nb)))
+                continue;
+            // so does this wave's part of a chunk whose own box is empty: the wave then only helps
+            // to stage the box for the others (uniform over the wave)
             bool taken = live;
-            if (kc.lds_texels && vol.occ && (pk[0] & 0xffffu) != 0xffffu) {
-                const int wo[3] = { int(pk[0] & 0xffffu), int(pk[0] >> 16), int(pk[2] & 0xffffu) };
-                const int wn[3] = { int(0xffffu - (pk[1] & 0xffffu)) - wo[0] + 1,
-                                    int(0xffffu - (pk[1] >> 16)) - wo[1] + 1,
-                                    int(0xffffu - (pk[2] >> 16)) - wo[2] + 1 };
-                if (boxEmpty(vol, wo, wn)) {
+            if (kc.lds_texels && vol.occ && wlo <= whi) {
+                int wo[3], wn[3];
+                if (!boxOf<ST>(wlo, whi, smin, smax, gc, kc.blo, kc.nhi, wo, wn) || boxEmpty(vol, wo, wn)) {
                     wlo = INT_MAX; whi = INT_MIN; wa = INT_MIN + 1; we = INT_MIN;
                     taken = false;
                 }
@@ -752,6 +763,7 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             const int pz = small ? px * nb[1] : 0;
             const int n_ins = small ? (pz * nb[2] + 127) >> 7 : 0; // 16-byte loads of 64 lanes = 128 texels
             if (small && n_ins * 128 <= int(kc.lds_texels)) {
+                __syncthreads(); // every wave is done with the previous chunk's box
                 // texel pairs t = 128 k + 2 lane of the box for k = wave, wave + 4, ...: position
                 // (cx, cy, cz) of the first by division, then steps of 512 texels with carries
                 int cx, cy, cz, dx, dy, dz;

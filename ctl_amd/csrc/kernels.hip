@@ -268,33 +268,43 @@ __device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, con
     while (r.i1 >= r.i0 && !insideQ(r.i1, r.s, c, blo, nhi)) --r.i1;
 }
 
-__device__ __forceinline__ int waveMin(int v)
+// Wave-wide min / max: DPP reductions within each 16-lane row, then the four row results. All
+// 64 lanes must be active.
+template <bool MAX>
+__device__ __forceinline__ int waveReduce(int v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
-    return __builtin_amdgcn_readfirstlane(v);
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false)); // row_ror:4
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false)); // row_ror:8
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
 }
 
-__device__ __forceinline__ int waveMax(int v)
+__device__ __forceinline__ int waveMin(int v) { return waveReduce<false>(v); }
+__device__ __forceinline__ int waveMax(int v) { return waveReduce<true>(v); }
+
+// Wave-wide minimum (maximum) of a float: DPP steps within each row of 16 lanes, then the four
+// rows' values. All 64 lanes must be active.
+template <bool MAX>
+__device__ __forceinline__ float waveReduceF(float v)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
-    return __builtin_amdgcn_readfirstlane(v);
+    auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+    v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0xB1, 0xF, 0xF, false)));
+    v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x4E, 0xF, 0xF, false)));
+    v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x124, 0xF, 0xF, false)));
+    v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x128, 0xF, 0xF, false)));
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return op(op(r0, r1), op(r2, r3));
 }
 
-__device__ __forceinline__ float waveMinF(float v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m, 64));
-    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
-}
-
-__device__ __forceinline__ float waveMaxF(float v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m, 64));
-    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
-}
+__device__ __forceinline__ float waveMinF(float v) { return waveReduceF<false>(v); }
+__device__ __forceinline__ float waveMaxF(float v) { return waveReduceF<true>(v); }
 
 // Adds this wave's lanes' counts to *tally (lane 0's atomic); all lanes active.
 __device__ __forceinline__ void tallyWave(unsigned long long* tally, uint32_t n)
@@ -516,21 +526,6 @@ __global__ __launch_bounds__(kBlock) void raycast_kernel(VolumeView vol, const R
 // Samples, their order within a ray and the FP32 arithmetic of each sample are the same as in
 // castTile. The box holds the volume's own 8-byte texels: FP32 (v, dz) (two 16-byte LDS reads per
 // sample, rows y and y+1) or FP16/BF16 (y,z)-quads (one 16-byte read).
-
-// Wave-wide min / max: DPP reductions within each 16-lane row, then the four row results. All
-// 64 lanes must be active.
-template <bool MAX>
-__device__ __forceinline__ int waveReduce(int v)
-{
-    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));  // quad_perm [1,0,3,2]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));  // quad_perm [2,3,0,1]
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false)); // row_ror:4
-    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false)); // row_ror:8
-    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
-    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
-    return op(op(r0, r1), op(r2, r3));
-}
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 

@@ -653,11 +653,41 @@ __device__ __forceinline__ bool boxOf(int s0, int s1, const float smin[3], const
     return any;
 }
 
+// LDS row pitch of a staged box nb[0] texels wide: even rows (a 16-byte load never straddles two
+// rows), not a multiple of 16 texels (rows would share LDS banks).
+__device__ __forceinline__ int boxPitch(int nbx)
+{
+    int px = (nbx + 1) & ~1;
+    if ((px & 15) == 0) px += 2;
+    return px;
+}
+
+// A box the staging indexes can address (a large volume stages through a window of the box's
+// slices, addressed with 32 bits).
+template <bool LARGE>
+__device__ __forceinline__ bool boxSmall(const int nb[3], const CastConst& kc)
+{
+    return nb[0] >= 1 && nb[0] <= 256 && nb[1] >= 1 && nb[1] <= 256 && nb[2] >= 1 && nb[2] <= 256 &&
+           (!LARGE || uint64_t(nb[2] + 1) * kc.pitch.tz * 8u < 0xffff0000ull);
+}
+
+// A box that is staged: addressable and within the LDS box.
+template <bool LARGE>
+__device__ __forceinline__ bool boxFits(const int nb[3], const CastConst& kc)
+{
+    if (!boxSmall<LARGE>(nb, kc)) return false;
+    const int n_ins = (boxPitch(nb[0]) * nb[1] * nb[2] + 127) >> 7;
+    return n_ins * 128 <= int(kc.lds_texels);
+}
+
+// ints per wave in the staged caster's exchange array
+constexpr int kRed = 10;
+
 template <Storage ST, int UNROLL, bool LARGE = false>
 __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayGeom* __restrict__ geom,
                                                const RayParams& rp, const CastConst& kc,
                                                float* __restrict__ out, uint32_t b, uint32_t wave,
-                                               float2* __restrict__ box, int (*red)[4][8])
+                                               float2* __restrict__ box, int (*red)[4][kRed])
 {
     const uint32_t vm = b / kc.tiles_per_vm;
     const uint32_t t = b - vm * kc.tiles_per_vm;
@@ -698,29 +728,37 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             smax[k] = waveMaxF(act ? ray.s[k] : -FLT_MAX);
         }
         if (lane == 0) {
-            red[1][wave][0] = r_lo;
-            red[1][wave][1] = r_hi;
+            red[2][wave][0] = r_lo;
+            red[2][wave][1] = r_hi;
+            red[2][wave][2] = r_alo;
+            red[2][wave][3] = r_ahi;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                red[1][wave][2 + k] = __float_as_int(smin[k]);
-                red[1][wave][5 + k] = __float_as_int(smax[k]);
+                red[2][wave][4 + k] = __float_as_int(smin[k]);
+                red[2][wave][7 + k] = __float_as_int(smax[k]);
             }
         }
         __syncthreads();
-        const int blo = min(min(red[1][0][0], red[1][1][0]), min(red[1][2][0], red[1][3][0]));
-        const int bhi = max(max(red[1][0][1], red[1][1][1]), max(red[1][2][1], red[1][3][1]));
+        const int blo = min(min(red[2][0][0], red[2][1][0]), min(red[2][2][0], red[2][3][0]));
+        const int bhi = max(max(red[2][0][1], red[2][1][1]), max(red[2][2][1], red[2][3][1]));
+        // the samples every active ray of the workgroup takes: in chunks inside this range the
+        // boxOf boxes are exactly the boxes of the lanes' taps (no lane's range starts or ends
+        // there); elsewhere the lanes' own boxes are reduced
+        const int g_alo = max(max(red[2][0][2], red[2][1][2]), max(red[2][2][2], red[2][3][2]));
+        const int g_ahi = min(min(red[2][0][3], red[2][1][3]), min(red[2][2][3], red[2][3][3]));
         float gmin[3], gmax[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            gmin[k] = fminf(fminf(__int_as_float(red[1][0][2 + k]), __int_as_float(red[1][1][2 + k])),
-                            fminf(__int_as_float(red[1][2][2 + k]), __int_as_float(red[1][3][2 + k])));
-            gmax[k] = fmaxf(fmaxf(__int_as_float(red[1][0][5 + k]), __int_as_float(red[1][1][5 + k])),
-                            fmaxf(__int_as_float(red[1][2][5 + k]), __int_as_float(red[1][3][5 + k])));
+            gmin[k] = fminf(fminf(__int_as_float(red[2][0][4 + k]), __int_as_float(red[2][1][4 + k])),
+                            fminf(__int_as_float(red[2][2][4 + k]), __int_as_float(red[2][3][4 + k])));
+            gmax[k] = fmaxf(fmaxf(__int_as_float(red[2][0][7 + k]), __int_as_float(red[2][1][7 + k])),
+                            fmaxf(__int_as_float(red[2][2][7 + k]), __int_as_float(red[2][3][7 + k])));
         }
         const int K = int(kc.chunk);
         float part[UNROLL];
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) part[u] = 0.0f;
+        int parity = 0; // half of red[0..1] the next box exchange writes
         for (int c0 = blo; c0 <= bhi; c0 += K) {
             const int c1 = min(c0 + K - 1, bhi);
             const int a = act ? max(c0, ray.i0) : INT_MAX;
@@ -734,31 +772,94 @@ __device__ __forceinline__ void castTileStaged(const VolumeView& vol, const RayG
             // the workgroup's texel box for the chunk (uniform): a chunk whose samples miss the
             // support, or whose box holds only zero voxels, adds exactly 0 to every ray
             int o[3] = { 0, 0, 0 }, nb[3] = { 0, 0, 0 };
-            if (kc.lds_texels && (!boxOf<ST>(c0, c1, gmin, gmax, gc, kc.blo, kc.nhi, o, nb) || boxEmpty(vol, o, nb)))
-                continue;
+            uint32_t pk[3] = { ~0u, ~0u, ~0u };
+            bool synced = false; // a workgroup barrier since the previous chunk's sampling
+            // the boxOf box: exact where no lane's range starts or ends in the chunk, else a
+            // superset, which serves unless it overflows the LDS box (uniform). Where rays enter
+            // or leave the volume at very different samples (oblique rays at the silhouette,
+            // coarse detector pixels) the superset spans the positions the late rays would have
+            // outside the volume along the other axes; the lanes' own boxes are then reduced.
+            // (64 projections of 64^3-256^3 volumes on 257^2-766^2 panels: 628 -> 585 ms; the
+            // 256^3 / 1024 x 768 scan 40.1 -> 40.6 ms)
+            bool exact = false;
+            if (kc.lds_texels) {
+                if (!boxOf<ST>(c0, c1, gmin, gmax, gc, kc.blo, kc.nhi, o, nb)) continue;
+                exact = !(c0 >= g_alo && c1 <= g_ahi) && !boxFits<LARGE>(nb, kc);
+            }
+            if (kc.lds_texels && !exact) {
+                if (boxEmpty(vol, o, nb)) continue;
+            } else if (kc.lds_texels) {
+                // Texel box of this lane's taps in the chunk, [lo, hi] per axis (texels carry z+1,
+                // and y+1 for the FP16/BF16 quads, so only x and for FP32 y need one more).
+                // Reduced over the wave as packed 16-bit minima (a maximum m as the minimum of
+                // 0xffff - m): padded sizes stay below 65535 when a box is staged. 0xffff marks
+                // "no live lane".
+                {
+                    const int as = live ? a : c0, es = live ? e : c0; // in range for every lane
+                    uint32_t lo[3], hi[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float qa = fmaf(float(as), ray.s[k], gc[k]), qe = fmaf(float(es), ray.s[k], gc[k]);
+                        const uint32_t extra = (k == 0 || (k == 1 && ST == Storage::F32)) ? 1u : 0u;
+                        lo[k] = uint32_t(int(fminf(qa, qe))) & 0xffffu;
+                        hi[k] = 0xffffu - ((uint32_t(int(fmaxf(qa, qe))) + extra) & 0xffffu);
+                    }
+                    pk[0] = live ? lo[0] | lo[1] << 16 : ~0u;
+                    pk[1] = live ? hi[0] | hi[1] << 16 : ~0u;
+                    pk[2] = live ? lo[2] | hi[2] << 16 : ~0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pk[k] = wavePkMin(pk[k]);
+                if (lane == 0)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) red[parity][wave][k] = int(pk[k]);
+                // the box of the previous chunk is no longer read; red[parity] is complete
+                __syncthreads();
+                synced = true;
+                uint32_t bpk[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    bpk[k] = pkMin(pkMin(uint32_t(red[parity][0][k]), uint32_t(red[parity][1][k])),
+                                   pkMin(uint32_t(red[parity][2][k]), uint32_t(red[parity][3][k])));
+                parity ^= 1; // the next exchange writes the other half (its readers passed this barrier)
+                o[0] = int(bpk[0] & 0xffffu);
+                o[1] = int(bpk[0] >> 16);
+                o[2] = int(bpk[2] & 0xffffu);
+                nb[0] = int(0xffffu - (bpk[1] & 0xffffu)) - o[0] + 1;
+                nb[1] = int(0xffffu - (bpk[1] >> 16)) - o[1] + 1;
+                nb[2] = int(0xffffu - (bpk[2] >> 16)) - o[2] + 1;
+                // 0xffff: no lane of the workgroup samples in this chunk
+                if (o[0] == 0xffff || boxEmpty(vol, o, nb)) continue;
+            }
             // so does this wave's part of a chunk whose own box is empty: the wave then only helps
             // to stage the box for the others (uniform over the wave)
             bool taken = live;
             if (kc.lds_texels && vol.occ && wlo <= whi) {
                 int wo[3], wn[3];
-                if (!boxOf<ST>(wlo, whi, smin, smax, gc, kc.blo, kc.nhi, wo, wn) || boxEmpty(vol, wo, wn)) {
+                bool empty;
+                if (!exact) {
+                    empty = !boxOf<ST>(wlo, whi, smin, smax, gc, kc.blo, kc.nhi, wo, wn) || boxEmpty(vol, wo, wn);
+                } else {
+                    wo[0] = int(pk[0] & 0xffffu);
+                    wo[1] = int(pk[0] >> 16);
+                    wo[2] = int(pk[2] & 0xffffu);
+                    wn[0] = int(0xffffu - (pk[1] & 0xffffu)) - wo[0] + 1;
+                    wn[1] = int(0xffffu - (pk[1] >> 16)) - wo[1] + 1;
+                    wn[2] = int(0xffffu - (pk[2] >> 16)) - wo[2] + 1;
+                    empty = wo[0] != 0xffff && boxEmpty(vol, wo, wn);
+                }
+                if (empty) {
                     wlo = INT_MAX; whi = INT_MIN; wa = INT_MIN + 1; we = INT_MIN;
                     taken = false;
                 }
             }
             if (kc.tally) tallyWave(kc.tally, taken ? uint32_t(e - a + 1) : 0u);
-            // a large volume stages through a window of the box's slices, addressed with 32 bits
-            const bool small = kc.lds_texels && nb[0] >= 1 && nb[0] <= 256 && nb[1] >= 1 && nb[1] <= 256 &&
-                               nb[2] >= 1 && nb[2] <= 256 &&
-                               (!LARGE || uint64_t(nb[2] + 1) * kc.pitch.tz * 8u < 0xffff0000ull);
-            // LDS pitches: even rows (a 16-byte load never straddles two rows), not a multiple of
-            // 16 texels (rows would share LDS banks)
-            int px = (nb[0] + 1) & ~1;
-            if ((px & 15) == 0) px += 2;
+            const bool small = kc.lds_texels && boxSmall<LARGE>(nb, kc);
+            const int px = boxPitch(nb[0]);
             const int pz = small ? px * nb[1] : 0;
             const int n_ins = small ? (pz * nb[2] + 127) >> 7 : 0; // 16-byte loads of 64 lanes = 128 texels
             if (small && n_ins * 128 <= int(kc.lds_texels)) {
-                __syncthreads(); // every wave is done with the previous chunk's box
+                if (!synced) __syncthreads(); // every wave is done with the previous chunk's box
                 // texel pairs t = 128 k + 2 lane of the box for k = wave, wave + 4, ...: position
                 // (cx, cy, cz) of the first by division, then steps of 512 texels with carries
                 int cx, cy, cz, dx, dy, dz;
@@ -859,7 +960,7 @@ __global__ __launch_bounds__(kBlock) void staged_kernel(VolumeView vol, const Ra
                                                         RayParams rp, CastConst kc, float* __restrict__ out)
 {
     extern __shared__ __attribute__((aligned(16))) float2 s_box[];
-    __shared__ int s_red[2][4][8];
+    __shared__ int s_red[3][4][kRed];
     static_assert(sizeof(s_red) <= kStaticLds, "static LDS beyond the reserve");
     castTileStaged<ST, UNROLL, LARGE>(vol, geom, rp, kc, out, xcdRemap(blockIdx.x, gridDim.x), threadIdx.x >> 6,
                                       s_box, s_red);
@@ -935,7 +1036,7 @@ __global__ __launch_bounds__(kBlock) void batch_staged_kernel(const BatchItem* _
                                                               uint32_t stride)
 {
     extern __shared__ __attribute__((aligned(16))) float2 s_box[];
-    __shared__ int s_red[2][4][8];
+    __shared__ int s_red[3][4][kRed];
     __shared__ unsigned long long s_tile;
     __shared__ uint32_t s_item;
     static_assert(sizeof(s_red) + 16 <= kStaticLds, "static LDS beyond the reserve");

@@ -112,8 +112,10 @@ def test_wait_timeout_and_drain(phantoms):
 
 def test_single_view_requests_pass_a_running_scan(phantoms):
     # a long scan cut into many launches, then DRRs: the DRRs run on the free launch slot beside
-    # the scan's launches and finish first, with the same result as alone; the scan is unaffected
-    srv = Server(hbm_budget=256 << 20, max_launch_pixels=1 << 18)
+    # the scan's launches and finish first, with the same result as alone; the scan is unaffected.
+    # Launches of 16 views, each several times a DRR's duration, so that the completion thread sees
+    # a DRR finish while a scan launch still runs.
+    srv = Server(hbm_budget=256 << 20, max_launch_pixels=1 << 20)
     vol = phantoms[0][1]
     scan_pm = scan_geometry(360, 256)
     drr_pms = [drr_geometry(300 + k) for k in range(4)]

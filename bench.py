@@ -37,8 +37,8 @@ ONCHIP_PEAK_GBPS = L1_PEAK_GBPS + LDS_PEAK_GBPS
 # (bench.py --copy-calibration: FETCH_SIZE 524315 KiB -> x2.0, WRITE_SIZE 1048576 KiB -> x1.0).
 # Chunks that read only zero voxels are skipped, so the kernel fetches about half the bytes it
 # fetches with every sample taken (5.11e7 KiB).
-FETCH_KIB, FETCH_CAL = 2.6079e7, 1048576 / 524315
-WRITE_KIB, WRITE_CAL = 1.1073e6, 1.0
+FETCH_KIB, FETCH_CAL = 2.6607e7, 1048576 / 524315
+WRITE_KIB, WRITE_CAL = 1.1499e6, 1.0
 TRAFFIC_F32_BYTES = (FETCH_KIB * FETCH_CAL + WRITE_KIB * WRITE_CAL) * 1024
 
 

@@ -120,14 +120,18 @@ uint32_t ldsTexels(const ctlproj_batch* b, const ctlk::LaunchShape& sh)
     return b->staged && sh.chunk ? sh.lds_texels : 0u;
 }
 
-// One timed launch over every `stride`-th tile (autotuner).
+// One timed launch over every `stride`-th run of ctlk::kSampleRun consecutive tiles (autotuner;
+// the whole runs that fit, or every tile with stride 1).
 float timedSample(ctlproj_batch* b, const ctlk::LaunchShape& sh, uint32_t stride)
 {
     if (buildItems(b, sh) != CTLPROJ_OK) return -1.0f;
     uint32_t grid = 0;
     if (ctlk::batchGrid(b->st, b->ly, b->interp, sh.unroll, sh.blocks_per_cu, ldsTexels(b, sh), &grid) != hipSuccess)
         return -1.0f;
-    const unsigned long long n = (b->total_tiles + stride - 1) / stride;
+    const unsigned long long span = (unsigned long long)stride * ctlk::kSampleRun;
+    if (stride > 1 && b->total_tiles < span) stride = 1;
+    const unsigned long long n =
+        stride == 1 ? b->total_tiles : b->total_tiles / ((unsigned long long)stride * ctlk::kSampleRun) * ctlk::kSampleRun;
     hipError_t e = ctlk::launchQueueInit(b->d_q, n, b->stream);
     if (e == hipSuccess) e = hipEventRecord(b->e0, b->stream);
     if (e == hipSuccess)

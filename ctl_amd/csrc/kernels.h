@@ -194,6 +194,9 @@ struct BatchItem {
     unsigned long long tile_begin;
 };
 
+// Tiles per run of the batch autotuner's sample (every stride-th run of consecutive tiles).
+constexpr uint32_t kSampleRun = 64;
+
 // Work queue of the persistent batch kernel: one tile range per XCD.
 struct BatchQueue {
     unsigned long long next[8];

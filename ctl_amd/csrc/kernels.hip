@@ -971,8 +971,14 @@ __global__ __launch_bounds__(kBlock) void staged_kernel(VolumeView vol, const Ra
 // 2x2 wavefronts of neighbouring pixels, so the four waves on a CU gather from the same patch
 // of the volume. The tile space is cut into one contiguous range per XCD; a workgroup takes tiles
 // from its own XCD's range first (neighbouring tiles of one view then share that XCD's L2) and
-// then from the others. `stride` > 1 visits every stride-th tile (used by the autotuner to time a
-// sample of the work).
+// then from the others. `stride` > 1 visits every stride-th run of kSampleRun consecutive tiles
+// (used by the autotuner to time a sample of the work; runs keep neighbouring tiles together, as a
+// full launch has them, so the sample's cache behaviour is the launch's).
+__device__ __forceinline__ unsigned long long sampledTile(unsigned long long t, uint32_t stride)
+{
+    return (t / kSampleRun) * stride * kSampleRun + t % kSampleRun;
+}
+
 // Takes the next tile for workgroup-home XCD `home` (own XCD's range first, then the others):
 // sets *t to the tile (~0 once every range is drained) and *k to its item. Called by all threads.
 __device__ __forceinline__ void grabTile(const BatchItem* __restrict__ items, uint32_t n_items,
@@ -990,7 +996,7 @@ __device__ __forceinline__ void grabTile(const BatchItem* __restrict__ items, ui
         }
         uint32_t lo = 0;
         if (got != ~0ull) {
-            const unsigned long long tt = got * stride;
+            const unsigned long long tt = sampledTile(got, stride);
             uint32_t hi = n_items; // last item with tile_begin <= tt
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) / 2;
@@ -1024,7 +1030,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const BatchItem* __restri
         if (t == ~0ull) break; // every workgroup exits once all ranges are drained
         const BatchItem& it = items[k];
         castTile<ST, LY, INTERP, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out,
-                                         uint32_t(t * stride - it.tile_begin), threadIdx.x >> 6);
+                                         uint32_t(sampledTile(t, stride) - it.tile_begin), threadIdx.x >> 6);
     }
 }
 
@@ -1047,7 +1053,7 @@ __global__ __launch_bounds__(kBlock) void batch_staged_kernel(const BatchItem* _
         grabTile(items, n_items, q, stride, home, &s_tile, &s_item, &t, &k);
         if (t == ~0ull) break;
         const BatchItem& it = items[k];
-        castTileStaged<ST, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out, uint32_t(t * stride - it.tile_begin),
+        castTileStaged<ST, UNROLL>(it.vol, it.geom, it.rp, it.kc, it.out, uint32_t(sampledTile(t, stride) - it.tile_begin),
                                    threadIdx.x >> 6, s_box, s_red);
     }
 }

@@ -259,6 +259,8 @@ __device__ __forceinline__ bool insideQ(int i, const float s[3], const float c[3
     return in;
 }
 
+// The walks run a step or two per end; kept as plain loops (the compiler's four-way unrolled
+// forms spilled registers to scratch in the staged caster and cost it 2%).
 __device__ __forceinline__ void tighten(Ray& r, const float c[3], float blo, const float nhi[3])
 {
     if (!r.hit) { r.i0 = 1; r.i1 = 0; return; }
